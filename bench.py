@@ -1,0 +1,146 @@
+"""Benchmark: FDK of a 512^3 volume from 720 views of 1024^2 on one MI355X.
+
+Workload (the projector-pair configuration at full size): 512^3 float32 volume,
+0.5 mm voxels, centred; 720 views over 360 deg, SID 1000 mm, SDD 1536 mm, 1024^2
+detector at 0.4 mm, per-view detector offsets jittered +-2 mm from the seed;
+projections are the analytic line integrals of a seeded 20-ellipsoid phantom on a
+0.02 mm^-1 base.  One step = one full FDK pass (fused cosine weighting + FFT ramp
+filter over the stack, then the voxel-driven FDK backprojection of all 720 views)
+with the projections already resident in HBM.
+
+value = voxel updates per second of the whole FDK step (GUPS, 512^3 * 720 updates per
+step).  roofline = the backprojection kernel against the HBM roofline with the
+algorithmic 16 bytes per voxel update (four FP32 taps), timed with HIP events on
+the stream it runs on.  cpu_baseline = the oracle's multithreaded FDK (a
+line-by-line restatement of RTK's CPU filters) on a bounded sample of the same
+workload (all voxels, first views only), timed on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+NX, NU, NVIEWS = 512, 1024, 720
+SEED = 20260001
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_workload(rtk, np):
+    rng = np.random.default_rng(SEED)
+    offs = rng.uniform(-2., 2., size=(NVIEWS, 2))
+    g = rtk.ThreeDCircularProjectionGeometry()
+    rows = []
+    for i in range(NVIEWS):
+        r = [1000., 1536., i * 360. / NVIEWS, offs[i, 0], offs[i, 1], 0., 0., 0., 0.]
+        g.AddProjection(*r)
+        rows.append(r)
+    half_fov = NX * 0.5 / 2
+    q = rtk.phantoms.random_ellipsoids(SEED, n=20, half_fov=half_fov)
+    det_origin = -(NU - 1) * 0.4 / 2
+    proj = rtk.project_quadrics(g, rtk.ConstantImageSource((NU, NU, NVIEWS), (0.4, 0.4, 1.),
+                                                          (det_origin, det_origin, 0.)), q)
+    vol = rtk.ConstantImageSource((NX, NX, NX), (0.5, 0.5, 0.5), (-(NX - 1) * 0.5 / 2,) * 3)
+    return np.array(rows), g, q, proj, vol
+
+
+def cpu_baseline(np, rows, proj, vol, views):
+    from oracle import rtkref
+    threads = min(16, os.cpu_count() or 1)
+    st = rtkref.image((NU, NU, views), proj.origin, proj.spacing)
+    ov = rtkref.image((NX,) * 3, vol.origin, vol.spacing)
+    p = proj.tensor[:views].cpu().numpy()
+    v = np.zeros((NX,) * 3, np.float32)
+    t0 = time.perf_counter()
+    rtkref.fdk(rows[:views], ov, v, st, p, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": round(NX ** 3 * views / dt / 1e9, 4), "unit": "GUPS", "cores": threads, "kind": "port",
+            "sample": f"FDK (weight+ramp+BP, float) of the full 512^3 volume from the first {views} of the 720 "
+                      f"views, {dt:.1f} s on {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--stack-dtype", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--cpu-views", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the single-GPU kernels run at the per-GPU share's full shape).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import rtk_amd as rtk
+
+    rtk.lib()
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    log("building workload (analytic projections of the 20-ellipsoid phantom)")
+    rows, g, q, proj, vol = build_workload(rtk, np)
+    torch.cuda.synchronize()
+
+    fdk = rtk.HipFDKConeBeamReconstructionFilter()
+    fdk.SetInput(0, vol)
+    fdk.SetInput(1, proj)
+    fdk.SetGeometry(g)
+    fdk.SetStackDtype(args.stack_dtype)
+    for w in range(args.warmup):
+        fdk.Update()
+        log(f"warmup {w + 1}/{args.warmup}: BP {fdk.GetBackProjectionTimeMs():.1f} ms")
+    torch.cuda.synchronize()
+    bp_ms = []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        fdk.Update()
+        bp_ms.append(fdk.GetBackProjectionTimeMs())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    for s, b in enumerate(bp_ms):
+        log(f"step {s + 1}: BP kernel {b:.1f} ms")
+    out = fdk.GetOutput().tensor
+    if not torch.isfinite(out).all():
+        raise RuntimeError("non-finite FDK output")
+
+    updates = float(NX) ** 3 * NVIEWS
+    ms_step = dt / args.steps * 1e3
+    bp_avg = float(np.mean(bp_ms))
+    bytes_per_update = 16 if args.stack_dtype == "f32" else 8
+    achieved = updates * bytes_per_update / (bp_avg * 1e-3) / 1e9
+    result = {
+        "metric": "FDK GUPS at 512^3 (weighting + FFT ramp + voxel backprojection, 720 views of 1024^2)",
+        "value": round(updates * args.steps / dt / 1e9, 3),
+        "unit": "GUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.stack_dtype == "f32" else f"f32 math, {args.stack_dtype} filtered stack",
+        "data": "synthetic: analytic FP64 projections of a seeded 20-ellipsoid phantom",
+        "config": {"workload": "FDK 512^3 @0.5 mm <- 720 x 1024^2 @0.4 mm, SID 1000, SDD 1536, +-2 mm offsets",
+                   "volume": [NX, NX, NX], "views": NVIEWS, "detector": [NU, NU],
+                   "bp_kernel_gups": round(updates / (bp_avg * 1e-3) / 1e9, 3),
+                   "bp_kernel_ms": round(bp_avg, 3)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(achieved / 8000.0, 4), "traffic": None},
+    }
+    if not args.no_cpu:
+        log(f"cpu baseline: oracle FDK on the first {args.cpu_views} views")
+        result["cpu_baseline"] = cpu_baseline(np, rows, proj, vol, args.cpu_views)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,250 @@
+/*
+ * rtkhip — C ABI of the MI355X (gfx950) projector hot path of RTK.
+ *
+ * Two levels:
+ *   1. Kernel level: the data each of RTK's CUDA back-end functions receives
+ *      (raw device pointers, int sizes, per-view matrices), so the reference's
+ *      CUDA filter classes could bind these in place of their CUDA_* calls.
+ *   2. Filter level: geometry handle + image descriptors, the work done by
+ *      RTK's filter classes on top of the kernels (matrix construction, FDK
+ *      subsets, ramp kernel, CG loop).  These are what rtk::Hip* filters and
+ *      the Python mirror call.
+ *
+ * Conventions
+ *   - Images are x-fastest (u-fastest for projection stacks: u, v, view), with
+ *     identity direction.  `index` is the buffered region's start index (ITK).
+ *   - Every function returns 0 (RTKHIP_OK) or an error code, never throws or
+ *     aborts; rtkhip_last_error() returns this thread's last message.
+ *   - `stream` is a hipStream_t (NULL = legacy default stream).  Kernels are
+ *     stream-ordered; filter-level entry points synchronise `stream` before
+ *     returning (drop-in semantics of the synchronous CUDA filters).
+ *   - Device pointers may alias where noted (in-place accumulation, like the
+ *     reference's InPlace filters).
+ */
+#ifndef RTKHIP_H
+#define RTKHIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+enum rtkhip_status
+{
+  RTKHIP_OK = 0,
+  RTKHIP_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad size, bad enum */
+  RTKHIP_ERR_GEOMETRY = 2,         /* geometry missing / fewer entries than views */
+  RTKHIP_ERR_UNSUPPORTED = 3,      /* e.g. cylindrical detector, parallel beam, > max views */
+  RTKHIP_ERR_HIP = 4,              /* HIP runtime error (launch, copy, ...) */
+  RTKHIP_ERR_OUT_OF_MEMORY = 5,    /* device allocation failed */
+  RTKHIP_ERR_NO_DEVICE = 6         /* no usable HIP device */
+};
+
+/* Pixel storage type of a projection stack read by a backprojector. */
+enum rtkhip_dtype
+{
+  RTKHIP_F32 = 0,
+  RTKHIP_F16 = 1,
+  RTKHIP_BF16 = 2
+};
+
+/* Border rule per view, chosen by the host exactly like RTK's CPU filters:
+ * RTKHIP_BORDER_OPTIMIZED = OptimizedBackprojectionX/Y (sample kept iff
+ * 0 <= floor(u) < Nu-1 and 0 <= floor(v) < Nv-1, plain bilinear);
+ * RTKHIP_BORDER_ITK_LINEAR = generic path with itk::LinearInterpolateImageFunction
+ * (inside iff -0.5 <= u < Nu-0.5, neighbours clamped). */
+enum rtkhip_border
+{
+  RTKHIP_BORDER_OPTIMIZED = 0,
+  RTKHIP_BORDER_ITK_LINEAR = 1
+};
+
+const char * rtkhip_last_error(void);
+const char * rtkhip_version(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+int rtkhip_device_count(void);
+
+/* ================================================================ kernel level */
+
+/* FDK voxel backprojection.  Replaces CUDA_reconstruct_conebeam
+ * (include/rtkCudaFDKBackProjectionImageFilter.hcu:22-28) and the CPU hot loop
+ * of include/rtkFDKBackProjectionImageFilter.hxx:232-306.
+ *   proj_dim   {Nu, Nv, Nviews}; vol_dim {Nx, Ny, Nz}
+ *   matrices   12 doubles per view: the 3x4 volume-index -> projection-index
+ *              matrix (u row, v row, w row; no transpose, no texture 0.5 offset),
+ *              already divided by the perspective factor at the rotation centre.
+ *   border     Nviews entries of enum rtkhip_border (NULL = all OPTIMIZED)
+ *   dev_vol_out = dev_vol_in + sum_views w^2 * bilinear(proj, u/w, v/w); in == out allowed;
+ *              dev_vol_in NULL = zeros.
+ *   proj_dtype enum rtkhip_dtype of dev_proj (FP32 math and accumulation in every case).
+ * Any number of views (the CUDA filter's 1024-view limit does not apply). */
+int rtkhip_fdk_backproject(const int proj_dim[3], const int vol_dim[3], const double * matrices,
+                           const int * border, const float * dev_vol_in, float * dev_vol_out, const void * dev_proj,
+                           int proj_dtype, void * stream);
+
+/* Plain voxel-driven backprojection (no weight).  Replaces CUDA_back_project
+ * (include/rtkCudaBackProjectionImageFilter.hcu:24-34) for a flat panel and the
+ * CPU loops of include/rtkBackProjectionImageFilter.hxx:352-475.  Same argument
+ * meaning as rtkhip_fdk_backproject, matrices not normalised. */
+int rtkhip_voxel_backproject(const int proj_dim[3], const int vol_dim[3], const double * matrices,
+                             const int * border, const float * dev_vol_in, float * dev_vol_out,
+                             const void * dev_proj, int proj_dtype, void * stream);
+
+/* Joseph forward projection.  Takes the data CUDA_forward_project receives
+ * (include/rtkCudaForwardProjectionImageFilter.hcu:24-38) but computes RTK's CPU
+ * Joseph algorithm (include/rtkJosephForwardProjectionImageFilter.hxx:182-543):
+ *   proj_to_vol  12 doubles per view: rows 0-2 of the 4x4 projection-index ->
+ *                volume-index matrix (third column unused)
+ *   source       3 doubles per view: source position in volume index coordinates
+ *   vol_spacing  3 doubles (mm per voxel; sets |stepMM|)
+ *   clip         inferior/superior clip along source->pixel (RTK defaults 0, 1)
+ *   dev_proj_out = dev_proj_in + integral; in == out allowed; dev_proj_in NULL = zeros.
+ *   samples      optional host pointer (NULL = not counted): receives the number
+ *                of Joseph samples (sum over rays of |fs-ns|+1); requesting it
+ *                synchronises `stream`. */
+int rtkhip_joseph_forward_project(const int proj_dim[3], const int vol_dim[3], const double * proj_to_vol,
+                                  const double * source, const double vol_spacing[3], double inferior_clip,
+                                  double superior_clip, const float * dev_proj_in, float * dev_proj_out,
+                                  const float * dev_vol, long long * samples, void * stream);
+
+/* FDK cosine/angular weighting.  Replaces CUDA_weight_projection
+ * (src/rtkCudaFDKWeightProjectionFilter.cu:86-129) with the CPU filter's math
+ * (include/rtkFDKWeightProjectionFilter.hxx:67-128).
+ *   geom7  7 doubles per view: sdd, sid, projOffsetX, projOffsetY, sourceOffsetY,
+ *          weight factor (angular gap * |sdd|/(2 sid^2) * |source|), tilt angle
+ *   proj_origin/proj_spacing: physical position of pixel (0,0) of the slab and pitch.
+ *   in == out allowed. */
+int rtkhip_fdk_weight(const int proj_dim[3], const double * geom7, const double proj_origin[2],
+                      const double proj_spacing[2], const float * dev_in, float * dev_out, void * stream);
+
+/* Ramp filtering of every row of a projection slab: zero padding to `xpad`
+ * (centred, optional Ohnesorge truncation mirror), R2C FFT, multiplication by
+ * the kernel spectrum, C2R with 1/xpad, crop.  Replaces CUDA_padding +
+ * CUDA_fft_convolution + CudaCropImageFilter
+ * (src/rtkCudaFFTProjectionsConvolutionImageFilter.cu:70-181) for 1-D kernels.
+ *   xpad       power of two, 2*Nu <= xpad <= 8192
+ *   kernel     (xpad/2+1) complex values as interleaved doubles (re, im)
+ *   trunc_next number of mirrored pixels per side (0 = none); mirror_weights has
+ *              trunc_next+1 doubles (weights[b] for border distance b)
+ *   weight_geom7  optional (NULL = none): fuse rtkhip_fdk_weight into the load
+ *   out_dtype  enum rtkhip_dtype of dev_out (F16/BF16 halve what the BP reads) */
+int rtkhip_ramp_filter(const int proj_dim[3], int xpad, const double * kernel, int trunc_next,
+                       const double * mirror_weights, const double * weight_geom7, const double proj_origin[2],
+                       const double proj_spacing[2], const float * dev_in, void * dev_out, int out_dtype,
+                       void * stream);
+
+/* One CG step of include/rtkConjugateGradientImageFilter.hxx:166-250 on device
+ * vectors of n floats, given AP = A*P (replaces CUDA_conjugate_gradient,
+ * src/rtkCudaConjugateGradientImageFilter.cu:80-122).  Scalars are FP64 and stay
+ * on the device: dev_state holds 4 doubles {rr, pAp, alpha, beta}; rr must hold
+ * ||R||^2 on entry (rtkhip_cg_init sets it) and holds ||R_new||^2 on exit.
+ *   alpha = rr / (pAp + eps); X += alpha P; R -= alpha AP; beta = rr_new / (rr + eps);
+ *   P = R + beta P.  eps = FLT_MIN (the reference's float eps). */
+int rtkhip_cg_step(size_t n, float * dev_x, float * dev_r, float * dev_p, const float * dev_ap,
+                   double * dev_state, void * stream);
+
+/* R = B - AX; P = R; dev_state[0] = ||R||^2 (FP64).  dev_r may alias dev_ax. */
+int rtkhip_cg_init(size_t n, const float * dev_b, const float * dev_ax, float * dev_r, float * dev_p,
+                   double * dev_state, void * stream);
+
+/* FP64 dot product of two device vectors (deterministic tree order); result written
+ * to *dev_result (device double). */
+int rtkhip_dot(size_t n, const float * dev_a, const float * dev_b, double * dev_result, void * stream);
+
+/* ================================================================ filter level */
+
+typedef struct rtkhip_image
+{
+  int    size[3];
+  int    index[3];
+  double origin[3];
+  double spacing[3];
+} rtkhip_image;
+
+/* Ramp options of rtk::FFTRampImageFilter (include/rtkFFTRampImageFilter.h).
+ * Only one window applies, in the order hann, cosine, hamming, ramlak, shepp_logan. */
+typedef struct rtkhip_ramp_options
+{
+  double hann_cut;
+  double hann_cut_y; /* must be 0 (2-D kernels are not supported yet) */
+  double cosine_cut;
+  double hamming;
+  double ramlak_cut;
+  double shepp_logan_cut;
+  double truncation_correction;
+  int    zero_pad_x; /* 1 or 2 (default 2) */
+} rtkhip_ramp_options;
+
+typedef struct rtkhip_geometry rtkhip_geometry;
+typedef struct rtkhip_context  rtkhip_context;
+
+/* rtk::ThreeDCircularProjectionGeometry
+ * (src/rtkThreeDCircularProjectionGeometry.cxx:55-153). Angles in degrees. */
+rtkhip_geometry * rtkhip_geometry_create(void);
+void              rtkhip_geometry_destroy(rtkhip_geometry * g);
+int rtkhip_geometry_add_projection(rtkhip_geometry * g, double sid, double sdd, double gantry_angle,
+                                   double proj_offset_x, double proj_offset_y, double out_of_plane_angle,
+                                   double in_plane_angle, double source_offset_x, double source_offset_y);
+int rtkhip_geometry_set_radius_cylindrical_detector(rtkhip_geometry * g, double radius);
+int rtkhip_geometry_size(const rtkhip_geometry * g);
+/* 3x4 matrix (row-major), source position (3) and source angle of view i. */
+int rtkhip_geometry_get(const rtkhip_geometry * g, int i, double matrix[12], double source[3], double * source_angle);
+
+/* A context owns cached device workspaces (matrix uploads, ramp kernels, CG
+ * vectors).  Use one context per host thread. */
+rtkhip_context * rtkhip_context_create(void);
+void             rtkhip_context_destroy(rtkhip_context * c);
+
+/* rtk::HipFDKConeBeamReconstructionFilter: weighting + ramp + FDK BP over
+ * projection subsets (include/rtkFDKConeBeamReconstructionFilter.hxx:116-157).
+ * vol_out = vol_in + FDK(projections).  subset_size <= 0 means all views at once.
+ * stack_dtype: storage of the filtered projections read by the BP (enum rtkhip_dtype).
+ * bp_ms (optional): milliseconds spent in the backprojection kernels. */
+int rtkhip_fdk_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const rtkhip_image * vol,
+                           const float * dev_vol_in, float * dev_vol_out, const rtkhip_image * stack,
+                           const float * dev_proj, const rtkhip_ramp_options * ramp, int subset_size,
+                           int stack_dtype, double * bp_ms, void * stream);
+
+/* rtk::HipForwardProjectionImageFilter (Joseph).  proj_out = proj_in + FP(vol). */
+int rtkhip_forward_project(rtkhip_context * c, const rtkhip_geometry * g, const rtkhip_image * stack,
+                           const float * dev_proj_in, float * dev_proj_out, const rtkhip_image * vol,
+                           const float * dev_vol, long long * samples, void * stream);
+
+/* rtk::HipBackProjectionImageFilter (voxel-based, fdk = 0) and the FDK
+ * backprojector (fdk = 1).  vol_out = vol_in + BP(proj). */
+int rtkhip_back_project(rtkhip_context * c, const rtkhip_geometry * g, const rtkhip_image * vol,
+                        const float * dev_vol_in, float * dev_vol_out, const rtkhip_image * stack,
+                        const void * dev_proj, int proj_dtype, int fdk, void * stream);
+
+/* rtk::HipConjugateGradientConeBeamReconstructionFilter with Joseph FP and voxel BP
+ * (include/rtkConjugateGradientConeBeamReconstructionFilter.hxx:195-305):
+ * solves BP(W FP(x)) = BP(W p) from dev_x (X0 on entry, result on exit).
+ *   dev_weights  optional projection weights (NULL = ones)
+ *   niter        maximum iterations
+ *   rel_tol      stop once ||B - A X_k|| / ||B|| <= rel_tol (0 = run niter); the
+ *                norm is the CG recursion's R_k, FP64
+ *   resid        optional host array of niter+1 doubles: ||R_k|| / ||B||
+ *   iterations   optional: iterations done */
+int rtkhip_cg_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const rtkhip_image * vol, float * dev_x,
+                          const rtkhip_image * stack, const float * dev_proj, const float * dev_weights, int niter,
+                          double rel_tol, double * resid, int * iterations, void * stream);
+
+/* Phantom generators (input generation for tests and benchmarks).  Each shape is
+ * 11 doubles: quadric coefficients A..J of A x^2 + B y^2 + C z^2 + D xy + E xz +
+ * F yz + G x + H y + I z + J <= 0 (src/rtkQuadricShape.cxx:30-41) and a density.
+ * draw: DrawConvexImageFilter (include/rtkDrawConvexImageFilter.hxx:41-61);
+ * project: RayConvexIntersectionImageFilter chain (FP64 analytic chords,
+ * include/rtkRayConvexIntersectionImageFilter.hxx:57-84). */
+int rtkhip_draw_quadrics(const rtkhip_image * vol, const float * dev_in, float * dev_out, int nshapes,
+                         const double * shapes, void * stream);
+int rtkhip_project_quadrics(rtkhip_context * c, const rtkhip_geometry * g, const rtkhip_image * stack,
+                            const float * dev_in, float * dev_out, int nshapes, const double * shapes,
+                            void * stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTKHIP_H */

@@ -1,0 +1,367 @@
+// FDK weighting and FFT ramp filtering for gfx950.
+//
+// Reference behaviour:
+//   weighting  include/rtkFDKWeightProjectionFilter.hxx:67-128
+//   ramp       include/rtkFFTProjectionsConvolutionImageFilter.hxx:121-276 (pad,
+//              R2C, multiply by the kernel row, C2R with 1/N, crop) and
+//              include/rtkFFTRampImageFilter.hxx:47-178 (kernel, built on the host)
+//
+// Design (CDNA4): one workgroup filters one detector row entirely in LDS.  The
+// zero-padded real row of length N is packed into N/2 complex points
+// (even + i odd), transformed by a radix-2 FFT in LDS, split into the N/2+1
+// half spectrum, multiplied by the kernel, re-packed and inverse transformed;
+// only the cropped centre is written.  The weighting is fused into the load
+// and the output may be written as FP16/BF16 for the backprojector, so the
+// stack crosses HBM once in and once out.
+#include "common.h"
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace rtkhip
+{
+namespace
+{
+
+// Per-view weighting constants (derived on the host in FP64 from the 7 values
+// of include/rtkFDKWeightProjectionFilter.hxx:85-116).
+struct WeightView
+{
+  float x0;       // physical x of pixel 0 + projOffsetX + tan(a) (sdd - sid)
+  float y0;       // physical y of row 0 + projOffsetY - sourceOffsetY
+  float numpart1; // sdd (cos a + tan a sin a)
+  float sina;
+  float sddtana;
+  float sdd2;
+  float factor; // angular-gap constant factor
+  float parallel;
+};
+
+__device__ __forceinline__ float
+CosGammaWeight(const WeightView & w, float dx, float dy, int i, int j)
+{
+  if (w.parallel != 0.f)
+    return w.factor;
+  const float px = w.x0 + dx * float(i);
+  const float py = w.y0 + dy * float(j);
+  const float t = px - w.sddtana;
+  const float denom = sqrtf(w.sdd2 + py * py + t * t);
+  return w.factor * (w.numpart1 - px * w.sina) / denom;
+}
+
+__global__ __launch_bounds__(256) void weight_kernel(const WeightView * __restrict__ views, int nu, int nv,
+                                                      float dx, float dy, const float * in, float * out)
+{
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int p = blockIdx.z;
+  if (i >= nu || j >= nv)
+    return;
+  const long o = (long(p) * nv + j) * nu + i;
+  out[o] = in[o] * CosGammaWeight(views[p], dx, dy, i, j);
+}
+
+__device__ __forceinline__ float2
+cmul(float2 a, float2 b)
+{
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2
+conj2(float2 a)
+{
+  return make_float2(a.x, -a.y);
+}
+
+__device__ __forceinline__ void
+StoreOut(float * p, long i, float v)
+{
+  p[i] = v;
+}
+__device__ __forceinline__ void
+StoreOut(__half * p, long i, float v)
+{
+  p[i] = __float2half(v);
+}
+__device__ __forceinline__ void
+StoreOut(__hip_bfloat16 * p, long i, float v)
+{
+  p[i] = __float2bfloat16(v);
+}
+
+// In-place radix-2 DIT FFT of M points in LDS whose input is stored in
+// bit-reversed order; tw[t] = exp(-2 pi i t / M).  INV uses conj(tw).
+template <bool INV>
+__device__ __forceinline__ void
+LdsFFT(float2 * buf, const float2 * tw, int M, int logM)
+{
+  for (int s = 0; s < logM; s++)
+  {
+    __syncthreads();
+    const int half = 1 << s;
+    for (int b = threadIdx.x; b < (M >> 1); b += blockDim.x)
+    {
+      const int pos = b & (half - 1);
+      const int i0 = ((b >> s) << (s + 1)) + pos;
+      const int i1 = i0 + half;
+      float2    w = tw[pos << (logM - 1 - s)];
+      if (INV)
+        w.y = -w.y;
+      const float2 a = buf[i0];
+      const float2 c = cmul(buf[i1], w);
+      buf[i0] = make_float2(a.x + c.x, a.y + c.y);
+      buf[i1] = make_float2(a.x - c.x, a.y - c.y);
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ int
+BitRev(int m, int logM)
+{
+  return int(__brev(unsigned(m)) >> (32 - logM));
+}
+
+// One workgroup per (row, view).  tables: [0, M/2) FFT twiddles exp(-2 pi i t/M),
+// [M/2, M/2+M+1) split twiddles exp(-2 pi i k/N), then kernel K[k] for k in [0, M].
+template <typename TO>
+__global__ __launch_bounds__(256) void ramp_kernel(const float * __restrict__ in, TO * __restrict__ out, int nu,
+                                                    int nv, int N, int logM, int zeroext,
+                                                    const float2 * __restrict__ tables, int next,
+                                                    const float * __restrict__ mirror,
+                                                    const WeightView * __restrict__ wviews, float dx, float dy)
+{
+  extern __shared__ float2 lds[];
+  const int M = N >> 1;
+  float2 *  buf = lds;          // M: forward transform
+  float2 *  buf2 = lds + M;     // M: inverse transform
+  float2 *  tw = lds + 2 * M;   // M/2
+  const int j = blockIdx.x;
+  const int p = blockIdx.y;
+  const float * row = in + (long(p) * nv + j) * nu;
+  TO *          orow = out + (long(p) * nv + j) * nu;
+  for (int t = threadIdx.x; t < (M >> 1); t += blockDim.x)
+    tw[t] = tables[t];
+  const WeightView * wv = wviews ? wviews + p : nullptr;
+  auto               W = [&](int i) -> float { return wv ? row[i] * CosGammaWeight(*wv, dx, dy, i, j) : row[i]; };
+  auto               X = [&](int n) -> float {
+    const int i = n - zeroext;
+    if (i >= 0 && i < nu)
+      return W(i);
+    if (next > 0)
+    {
+      if (i < 0 && -i <= next)
+        return mirror[-i] * (2.f * W(1) - W(-i));
+      if (i >= nu && i - (nu - 1) <= next)
+      {
+        const int b = i - (nu - 1);
+        return mirror[b] * (2.f * W(nu - 1) - W(nu - 1 - b));
+      }
+    }
+    return 0.f;
+  };
+  for (int m = threadIdx.x; m < M; m += blockDim.x)
+    buf[BitRev(m, logM)] = make_float2(X(2 * m), X(2 * m + 1));
+  LdsFFT<false>(buf, tw, M, logM);
+
+  // Split to the half spectrum, multiply by the kernel, re-pack for the inverse
+  // into the second buffer (bit-reversed for the inverse DIT FFT).
+  const float2 * split = tables + (M >> 1);
+  const float2 * K = split + (M + 1);
+  const int      npairs = (M >> 1) + 1; // k = 0 .. M/2, partner M-k
+  for (int k = threadIdx.x; k < npairs; k += blockDim.x)
+  {
+    const int    km = (M - k) & (M - 1);
+    const float2 Zk = buf[k & (M - 1)], Zm = buf[km];
+    // X[k] = (Zk + conj Zm)/2 + W^k (Zk - conj Zm)/(2i), and the same for M-k.
+    auto Xof = [&](float2 A, float2 B, float2 w) {
+      const float2 e = make_float2(0.5f * (A.x + B.x), 0.5f * (A.y - B.y));
+      const float2 o = make_float2(0.5f * (A.y + B.y), -0.5f * (A.x - B.x));
+      const float2 wo = cmul(w, o);
+      return make_float2(e.x + wo.x, e.y + wo.y);
+    };
+    float2 Xk, Xm;
+    if (k == 0)
+    {
+      Xk = make_float2(Zk.x + Zk.y, 0.f); // X[0]
+      Xm = make_float2(Zk.x - Zk.y, 0.f); // X[M] (Nyquist)
+    }
+    else
+    {
+      Xk = Xof(Zk, Zm, split[k]);
+      Xm = Xof(Zm, Zk, split[M - k]);
+    }
+    const float2 Yk = cmul(Xk, K[k]);
+    const float2 Ym = cmul(Xm, K[M - k]);
+    // Z'[k] = Ye + i Yo, Ye = (Y[k] + conj Y[M-k])/2, Yo = (Y[k] - conj Y[M-k]) conj(W^k)/2.
+    auto Zof = [&](float2 A, float2 B, float2 w) {
+      const float2 e = make_float2(0.5f * (A.x + B.x), 0.5f * (A.y - B.y));
+      const float2 d = make_float2(0.5f * (A.x - B.x), 0.5f * (A.y + B.y));
+      const float2 o = cmul(d, conj2(w));
+      return make_float2(e.x - o.y, e.y + o.x);
+    };
+    if (k < M)
+      buf2[BitRev(k, logM)] = Zof(Yk, Ym, split[k]);
+    const int mk = M - k;
+    if (mk != k && mk < M)
+      buf2[BitRev(mk, logM)] = Zof(Ym, Yk, split[mk]);
+  }
+  LdsFFT<true>(buf2, tw, M, logM);
+  const float scale = 1.f / float(M);
+  for (int i = threadIdx.x; i < nu; i += blockDim.x)
+  {
+    const int    n = zeroext + i;
+    const float2 z = buf2[n >> 1];
+    StoreOut(orow, i, ((n & 1) ? z.y : z.x) * scale);
+  }
+}
+
+std::vector<WeightView>
+MakeWeightViews(int np, const double * g7, const double origin[2])
+{
+  std::vector<WeightView> v(np);
+  for (int p = 0; p < np; p++)
+  {
+    const double * g = g7 + 7 * p;
+    const double   sdd = g[0], sid = g[1], pox = g[2], poy = g[3], soy = g[4], fac = g[5], tilt = g[6];
+    WeightView &   w = v[p];
+    w.factor = float(fac);
+    if (sdd == 0.)
+    {
+      w.parallel = 1.f;
+      continue;
+    }
+    const double cosa = std::cos(tilt), sina = std::sin(tilt), tana = std::tan(tilt);
+    w.parallel = 0.f;
+    w.x0 = float(origin[0] + pox + tana * (sdd - sid));
+    w.y0 = float(origin[1] + poy - soy);
+    w.numpart1 = float(sdd * (cosa + tana * sina));
+    w.sina = float(sina);
+    w.sddtana = float(sdd * tana);
+    w.sdd2 = float(sdd * sdd);
+  }
+  return v;
+}
+
+struct RampCache
+{
+  std::map<hipStream_t, DeviceBuffer<WeightView>> views;
+  std::map<hipStream_t, DeviceBuffer<float2>>     tables;
+  std::map<hipStream_t, DeviceBuffer<float>>      mirror;
+};
+thread_local RampCache g_ramp;
+
+} // namespace
+} // namespace rtkhip
+
+extern "C" int
+rtkhip_fdk_weight(const int proj_dim[3], const double * geom7, const double proj_origin[2],
+                  const double proj_spacing[2], const float * dev_in, float * dev_out, void * stream)
+{
+  using namespace rtkhip;
+  return Guard([&] {
+    RTKHIP_REQUIRE(proj_dim && geom7 && proj_origin && proj_spacing && dev_in && dev_out,
+                   RTKHIP_ERR_INVALID_ARGUMENT, "fdk_weight: null argument");
+    const int nu = proj_dim[0], nv = proj_dim[1], np = proj_dim[2];
+    RTKHIP_REQUIRE(nu > 0 && nv > 0 && np >= 0 && np <= 65535, RTKHIP_ERR_INVALID_ARGUMENT,
+                   "fdk_weight: bad sizes");
+    RequireDevice();
+    if (np == 0)
+      return;
+    hipStream_t s = Stream(stream);
+    auto        hv = MakeWeightViews(np, geom7, proj_origin);
+    auto &      dv = g_ramp.views[s];
+    dv.Resize(np);
+    RTKHIP_CHECK(hipMemcpyAsync(dv.get(), hv.data(), sizeof(WeightView) * np, hipMemcpyHostToDevice, s));
+    dim3 grid((nu + 63) / 64, (nv + 3) / 4, np);
+    hipLaunchKernelGGL(weight_kernel, grid, dim3(256), 0, s, dv.get(), nu, nv, float(proj_spacing[0]),
+                       float(proj_spacing[1]), dev_in, dev_out);
+    RTKHIP_CHECK_LAUNCH();
+  });
+}
+
+extern "C" int
+rtkhip_ramp_filter(const int proj_dim[3], int xpad, const double * kernel, int trunc_next,
+                   const double * mirror_weights, const double * weight_geom7, const double proj_origin[2],
+                   const double proj_spacing[2], const float * dev_in, void * dev_out, int out_dtype,
+                   void * stream)
+{
+  using namespace rtkhip;
+  return Guard([&] {
+    RTKHIP_REQUIRE(proj_dim && kernel && dev_in && dev_out, RTKHIP_ERR_INVALID_ARGUMENT, "ramp_filter: null argument");
+    const int nu = proj_dim[0], nv = proj_dim[1], np = proj_dim[2];
+    RTKHIP_REQUIRE(nu > 1 && nv > 0 && np >= 0 && np <= 65535 && nv <= 2147483647, RTKHIP_ERR_INVALID_ARGUMENT,
+                   "ramp_filter: bad sizes");
+    RTKHIP_REQUIRE(xpad >= 4 && xpad <= 8192 && (xpad & (xpad - 1)) == 0 && xpad >= nu, RTKHIP_ERR_UNSUPPORTED,
+                   "ramp_filter: padded size must be a power of two in [max(4, Nu), 8192]");
+    RTKHIP_REQUIRE(trunc_next >= 0 && (trunc_next == 0 || (mirror_weights && trunc_next < nu - 1)),
+                   RTKHIP_ERR_INVALID_ARGUMENT, "ramp_filter: bad truncation arguments");
+    RTKHIP_REQUIRE(out_dtype == RTKHIP_F32 || out_dtype == RTKHIP_F16 || out_dtype == RTKHIP_BF16,
+                   RTKHIP_ERR_INVALID_ARGUMENT, "ramp_filter: unknown output dtype");
+    RTKHIP_REQUIRE(!weight_geom7 || (proj_origin && proj_spacing), RTKHIP_ERR_INVALID_ARGUMENT,
+                   "ramp_filter: fused weighting needs origin and spacing");
+    RTKHIP_REQUIRE((dev_in != dev_out) || out_dtype == RTKHIP_F32, RTKHIP_ERR_INVALID_ARGUMENT,
+                   "ramp_filter: in-place filtering needs an FP32 output");
+    RequireDevice();
+    if (np == 0)
+      return;
+    hipStream_t s = Stream(stream);
+    const int   M = xpad / 2;
+    int         logM = 0;
+    while ((1 << logM) < M)
+      logM++;
+    const long zeroext = (long(xpad) - nu) / 2;
+    RTKHIP_REQUIRE(zeroext >= trunc_next, RTKHIP_ERR_INVALID_ARGUMENT, "ramp_filter: mirror wider than padding");
+    const double pi = 3.141592653589793238462643383279502884;
+    std::vector<float2> tab(M / 2 + (M + 1) + (M + 1));
+    for (int t = 0; t < M / 2; t++)
+      tab[t] = make_float2(float(std::cos(-2 * pi * t / M)), float(std::sin(-2 * pi * t / M)));
+    for (int k = 0; k <= M; k++)
+      tab[M / 2 + k] = make_float2(float(std::cos(-2 * pi * k / xpad)), float(std::sin(-2 * pi * k / xpad)));
+    for (int k = 0; k <= M; k++)
+      tab[M / 2 + M + 1 + k] = make_float2(float(kernel[2 * k]), float(kernel[2 * k + 1]));
+    auto & dt = g_ramp.tables[s];
+    dt.Resize(tab.size());
+    RTKHIP_CHECK(hipMemcpyAsync(dt.get(), tab.data(), sizeof(float2) * tab.size(), hipMemcpyHostToDevice, s));
+    const float * dmirror = nullptr;
+    if (trunc_next > 0)
+    {
+      std::vector<float> mw(trunc_next + 1);
+      for (int b = 0; b <= trunc_next; b++)
+        mw[b] = float(mirror_weights[b]);
+      auto & dm = g_ramp.mirror[s];
+      dm.Resize(mw.size());
+      RTKHIP_CHECK(hipMemcpyAsync(dm.get(), mw.data(), sizeof(float) * mw.size(), hipMemcpyHostToDevice, s));
+      dmirror = dm.get();
+    }
+    const WeightView * dw = nullptr;
+    float              dx = 0.f, dy = 0.f;
+    if (weight_geom7)
+    {
+      auto   hv = MakeWeightViews(np, weight_geom7, proj_origin);
+      auto & dv = g_ramp.views[s];
+      dv.Resize(np);
+      RTKHIP_CHECK(hipMemcpyAsync(dv.get(), hv.data(), sizeof(WeightView) * np, hipMemcpyHostToDevice, s));
+      dw = dv.get();
+      dx = float(proj_spacing[0]);
+      dy = float(proj_spacing[1]);
+    }
+    const size_t lds = sizeof(float2) * (2 * M + M / 2);
+    dim3         grid(nv, np);
+#define RTKHIP_RAMP_LAUNCH(TO)                                                                                       \
+  hipLaunchKernelGGL(ramp_kernel<TO>, grid, dim3(256), lds, s, dev_in, static_cast<TO *>(dev_out), nu, nv, xpad,    \
+                     logM, int(zeroext), dt.get(), trunc_next, dmirror, dw, dx, dy)
+    if (out_dtype == RTKHIP_F32)
+      RTKHIP_RAMP_LAUNCH(float);
+    else if (out_dtype == RTKHIP_F16)
+      RTKHIP_RAMP_LAUNCH(__half);
+    else
+      RTKHIP_RAMP_LAUNCH(__hip_bfloat16);
+#undef RTKHIP_RAMP_LAUNCH
+    RTKHIP_CHECK_LAUNCH();
+  });
+}

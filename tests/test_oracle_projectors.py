@@ -1,0 +1,140 @@
+"""Pins the oracle's projectors.
+
+Backprojection: an independent brute-force backprojector written in physical
+coordinates (project each voxel centre with the documented 3x4 matrix, convert
+to a detector index, bilinear with the CPU border rule) must agree with the
+oracle, which follows RTK's transposed index-to-index matrices and incremental
+Y-path loops.  Constant-1 projections give per-voxel view counts (plain) and
+sums of w^2 (FDK).
+
+Joseph: a constant-1 volume gives the exact chord of the voxel-centre box,
+clipped to the pixel-source segment (inner and outer sources); the Shepp-Logan
+case of test/rtkforwardprojectiontest.cxx:153-180 must meet its thresholds.
+"""
+import numpy as np
+import pytest
+
+from helpers import check_image_quality, circular_rows
+from test_oracle_geometry import doc_matrix
+
+
+def naive_bp(rows, vsize, vorig, vsp, ssize, sorig, ssp, proj, fdk):
+    nx, ny, nz = vsize
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    X = np.stack([vorig[0] + vsp[0] * i, vorig[1] + vsp[1] * j, vorig[2] + vsp[2] * k, np.ones_like(i, float)], -1)
+    out = np.zeros((nz, ny, nx))
+    nu, nv = ssize[0], ssize[1]
+    for p, r in enumerate(rows):
+        M = doc_matrix(*r)
+        h = X @ M.T
+        u = ((h[..., 0] / h[..., 2]) - sorig[0]) / ssp[0]
+        v = ((h[..., 1] / h[..., 2]) - sorig[1]) / ssp[1]
+        ui, vi = np.floor(u).astype(int), np.floor(v).astype(int)
+        ok = (ui >= 0) & (ui < nu - 1) & (vi >= 0) & (vi < nv - 1)
+        uc, vc = np.clip(ui, 0, nu - 2), np.clip(vi, 0, nv - 2)
+        fu, fv = u - ui, v - vi
+        P = proj[p]
+        val = ((1 - fu) * (1 - fv) * P[vc, uc] + fu * (1 - fv) * P[vc, uc + 1] + (1 - fu) * fv * P[vc + 1, uc]
+               + fu * fv * P[vc + 1, uc + 1])
+        if fdk:
+            w0 = (M @ np.array([0, 0, 0, 1.]))[2]
+            val = val * (w0 / h[..., 2]) ** 2
+        out += np.where(ok, val, 0.)
+    return out
+
+
+GEOMS = {
+    "circular": circular_rows(9, 600., 1000., arc=360., start=7.),
+    "offsets": np.array([[600., 1000., 40 * i + 3, 2.5 * i - 4, -3. + i, 0, 0, 5. - i, 2.]
+                         for i in range(9)]),
+}
+
+
+@pytest.mark.parametrize("fdk", [False, True])
+@pytest.mark.parametrize("gname", list(GEOMS))
+def test_backproject_matches_brute_force(oracle, fdk, gname):
+    rows = GEOMS[gname]
+    vsize, vsp = (13, 11, 9), (3., 2.5, 3.5)
+    vorig = (-17., -13., -14.)
+    ssize, ssp, sorig = (21, 17, len(rows)), (4., 4.), (-38., -30.)
+    rng = np.random.default_rng(7)
+    proj = rng.normal(size=(len(rows), 17, 21))
+    vol = oracle.image(vsize, vorig, vsp)
+    st = oracle.image(ssize, sorig + (0.,), ssp + (1.,))
+    got = oracle.backproject(rows, vol, np.zeros((9, 11, 13)), st, proj, fdk=fdk)
+    ref = naive_bp(rows, vsize, vorig, vsp, ssize, sorig, ssp, proj, fdk)
+    np.testing.assert_allclose(got, ref, rtol=1e-9, atol=1e-9)
+
+
+def test_backproject_constant_counts(oracle):
+    rows = circular_rows(24, 500., 800.)
+    vsize = (16, 16, 16)
+    vol = oracle.image(vsize, (-30, -30, -30), (4, 4, 4))
+    st = oracle.image((20, 12, 24), (-38, -22, 0), (4, 4, 1))
+    ones = np.ones((24, 12, 20), np.float32)
+    got = oracle.backproject(rows, vol, np.zeros((16, 16, 16), np.float32), st, ones)
+    ref = naive_bp(rows, vsize, (-30,) * 3, (4,) * 3, (20, 12), (-38, -22), (4, 4), ones.astype(float), False)
+    # constant projections: every kept sample interpolates to 1, so voxels hold integer view counts
+    assert np.all(np.abs(got - np.round(ref)) < 1e-5)
+    assert got.max() == 24 and got.min() < 24
+
+
+def box_chord(src, pix, lo, hi):
+    d = pix - src
+    t0, t1 = 0., 1.
+    for i in range(3):
+        if d[i] == 0:
+            if not (lo[i] <= src[i] <= hi[i]):
+                return 0.
+            continue
+        a, b = (lo[i] - src[i]) / d[i], (hi[i] - src[i]) / d[i]
+        t0, t1 = max(t0, min(a, b)), min(t1, max(a, b))
+    return max(0., t1 - t0) * np.linalg.norm(d)
+
+
+@pytest.mark.parametrize("sid", [500., 47.6])
+def test_joseph_constant_volume_is_box_chord(oracle, sid):
+    # rtkforwardprojectiontest cases 1-2: a 64^3 all-ones volume, 4 mm voxels.
+    n = 12
+    rows = np.array([[sid / np.cos(np.deg2rad(a)) if sid < 100 else sid, 1000., q * 90 + a, 0, 0, 0, 0, 0, 0]
+                     for q, a in [(i % 4, -45. + 7 * i) for i in range(n)]])
+    vol = oracle.image((64, 64, 64), (-126,) * 3, (4,) * 3)
+    st = oracle.image((64, 64, n), (-126, -126, 0), (4, 4, 1))
+    p = oracle.joseph(rows, st, np.zeros((n, 64, 64)), vol, np.ones((64, 64, 64)))
+    m, s, *_ = oracle.geometry(rows)
+    rng = np.random.default_rng(0)
+    for _ in range(60):
+        k, jv, iu = rng.integers(n), rng.integers(64), rng.integers(64)
+        r = rows[k]
+        # pixel position: R^T (u + px, v + py, sid - sdd)
+        from test_oracle_geometry import _rx, _ry, _rz
+        R = _rz(-np.deg2rad(r[6])) @ _rx(-np.deg2rad(r[5])) @ _ry(-np.deg2rad(r[2]))
+        pix = R.T @ np.array([-126 + 4 * iu + r[3], -126 + 4 * jv + r[4], r[0] - r[1]])
+        ref = box_chord(s[k], pix, [-126.] * 3, [126.] * 3)
+        assert abs(p[k, jv, iu] - ref) < 1e-9 * max(1., ref), (k, jv, iu)
+
+
+def test_joseph_shepp_logan_vs_analytic(oracle):
+    # test/rtkforwardprojectiontest.cxx:153-180 (outer source, 45 views, 2 mm voxels).
+    from rtk_amd import phantoms
+    n = 45
+    rows = np.array([[500., 1000., i * 8., 0, 0, 0, 0, 0, 0] for i in range(n)])
+    st = oracle.image((64, 64, n), (-126, -126, -126), (4, 4, 4))
+    q = phantoms.shepp_logan(128.)
+    ref = oracle.project_quadrics(rows, st, np.zeros((n, 64, 64), np.float32), q)
+    vol = oracle.image((128, 128, 128), (-127,) * 3, (2,) * 3)
+    v = oracle.draw_quadrics(vol, np.zeros((128, 128, 128), np.float32), q)
+    fp = oracle.joseph(rows, st, np.zeros((n, 64, 64), np.float32), vol, v)
+    err, psnr, ok = check_image_quality(fp, ref, 1.28, 44., 255.)
+    assert ok, (err, psnr)
+
+
+def test_joseph_float_close_to_double(oracle):
+    rows = circular_rows(8, 600., 1000.)
+    rng = np.random.default_rng(3)
+    v = rng.random((20, 24, 22))
+    vol = oracle.image((22, 24, 20), (-20, -23, -19), (2, 2, 2))
+    st = oracle.image((30, 26, 8), (-58, -50, 0), (4, 4, 1))
+    pd = oracle.joseph(rows, st, np.zeros((8, 26, 30)), vol, v)
+    pf = oracle.joseph(rows, st, np.zeros((8, 26, 30), np.float32), vol, v.astype(np.float32))
+    assert np.linalg.norm(pf - pd) / np.linalg.norm(pd) < 1e-6
