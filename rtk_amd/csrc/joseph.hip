@@ -125,6 +125,7 @@ __global__ __launch_bounds__(256) void joseph_kernel(const JosephView * __restri
         dSup = t;
       }
       const double maxx = bmax[dInf], maxy = bmax[dSup];
+      const int    nxm2 = n3[dInf] - 2, nym2 = n3[dSup] - 2;
       const long   ox = off3[dInf], oy = off3[dSup];
       long         oz = off3[mainDir];
       const float * b = vol + ns * oz;
@@ -146,7 +147,8 @@ __global__ __launch_bounds__(256) void joseph_kernel(const JosephView * __restri
 
       // BilinearInterpolationOnBorders (.hxx:493-543): inward clamp at the box faces.
       auto border = [&](double step, const float * bb, double x, double y) -> float {
-        const int    ix = int(floor(x)), iy = int(floor(y));
+        // |residual * step| <= 0.5 keeps floor in [-1, N-1]; the clamp guards rounding.
+        const int    ix = min(max(int(floor(x)), -1), nxm2 + 1), iy = min(max(int(floor(y)), -1), nym2 + 1);
         const long   idx = ix * ox + iy * oy;
         const double lx = x - ix, ly = y - iy, lxc = 1. - lx, lyc = 1. - ly;
         const long   xi = (ix < 0) ? ox : 0, yi = (iy < 0) ? oy : 0;
@@ -170,8 +172,10 @@ __global__ __launch_bounds__(256) void joseph_kernel(const JosephView * __restri
         const int nmid = abs(fs - ns) - 1;
         for (int t = 0; t < nmid; ++t)
         {
-          // BilinearInterpolation (.hxx:443-477)
-          const int    ix = int(floor(cx)), iy = int(floor(cy));
+          // BilinearInterpolation (.hxx:443-477).  Middle samples lie on the
+          // segment inside the box; the clamp only guards grazing rays whose
+          // rounded position leaves [0, N-2] (the CPU would read outside).
+          const int    ix = min(max(int(floor(cx)), 0), nxm2), iy = min(max(int(floor(cy)), 0), nym2);
           const long   idx = ix * ox + iy * oy;
           const double lx = cx - ix, ly = cy - iy, lxc = 1. - lx, lyc = 1. - ly;
           const float  v00 = b[idx], v10 = b[idx + ox], v01 = b[idx + oy], v11 = b[idx + ox + oy];

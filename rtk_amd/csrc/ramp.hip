@@ -351,6 +351,12 @@ rtkhip_ramp_filter(const int proj_dim[3], int xpad, const double * kernel, int t
       dy = float(proj_spacing[1]);
     }
     const size_t lds = sizeof(float2) * (2 * M + M / 2);
+    int          dev = 0, maxLds = 0;
+    RTKHIP_CHECK(hipGetDevice(&dev));
+    RTKHIP_CHECK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    RTKHIP_REQUIRE(lds <= size_t(maxLds), RTKHIP_ERR_UNSUPPORTED,
+                   "ramp_filter: padded row does not fit in LDS (" + std::to_string(lds) + " > " +
+                     std::to_string(maxLds) + " bytes)");
     dim3         grid(nv, np);
 #define RTKHIP_RAMP_LAUNCH(TO)                                                                                       \
   hipLaunchKernelGGL(ramp_kernel<TO>, grid, dim3(256), lds, s, dev_in, static_cast<TO *>(dev_out), nu, nv, xpad,    \

@@ -87,7 +87,7 @@ def test_in_place_and_empty_stack(oracle, hip):
     assert f.GetOutput().tensor.data_ptr() == vol.tensor.data_ptr()
     # constant-1 projections add integer view counts
     d = vol.numpy() - v0
-    assert np.all(np.abs(d - np.round(d)) < 2e-4) and d.max() == 23
+    assert np.all(np.abs(d - np.round(d)) < 2e-4) and abs(d.max() - 23) < 2e-4
     # zero views: output equals input
     empty = to_device(hip, np.zeros((1, 47, 61)), (-96., -73., 0.), (3.2, 3.2, 1.))
     empty.tensor = empty.tensor[:0]
