@@ -65,6 +65,48 @@ def cpu_baseline(np, rows, proj, vol, views):
                       f"views, {dt:.1f} s on {threads} threads"}
 
 
+def cg_measurement(rtk, np, torch, max_iter, tol):
+    """Plain CG (Joseph FP + voxel BP, FP64 scalars) on 256^3 @1 mm from 360 views of
+    512^2 @0.8 mm with Poisson noise (I0 = 1e4), from zero and from the FDK volume."""
+    n, nu, nx = 360, 512, 256
+    g = rtk.ThreeDCircularProjectionGeometry()
+    for i in range(n):
+        g.AddProjection(1000., 1536., i * 360. / n)
+    q = rtk.phantoms.random_ellipsoids(SEED + 3, n=20, half_fov=nx * 1.0 / 2)
+    det0 = -(nu - 1) * 0.8 / 2
+    clean = rtk.project_quadrics(g, rtk.ConstantImageSource((nu, nu, n), (0.8, 0.8, 1.), (det0, det0, 0.)), q)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(SEED)
+    counts = torch.poisson(1e4 * torch.exp(-clean.tensor.double()), generator=gen)
+    proj = clean.like((-torch.log(torch.clamp(counts, min=1.) / 1e4)).float().contiguous())
+    vol = rtk.ConstantImageSource((nx, nx, nx), (1., 1., 1.), (-(nx - 1) / 2.,) * 3)
+    out = {}
+    fdk = rtk.HipFDKConeBeamReconstructionFilter()
+    fdk.SetInput(0, vol)
+    fdk.SetInput(1, proj)
+    fdk.SetGeometry(g)
+    fdk.Update()
+    for name, x0 in [("zero", vol), ("fdk", fdk.GetOutput())]:
+        cg = rtk.HipConjugateGradientConeBeamReconstructionFilter()
+        cg.SetInput(0, x0)
+        cg.SetInput(1, proj)
+        cg.SetGeometry(g)
+        cg.SetNumberOfIterations(max_iter)
+        cg.SetResidualTolerance(tol)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cg.Update()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res = cg.GetResiduals()
+        out[name] = {"seconds": round(dt, 3), "iterations": int(cg.GetNumberOfIterationsDone()),
+                     "final_rel_residual": float(res[-1]), "reached": bool(res[-1] <= tol),
+                     "ms_per_iteration": round(dt * 1e3 / max(1, cg.GetNumberOfIterationsDone()), 2)}
+        log(f"cg from {name}: {out[name]}")
+    return {"workload": "CG 256^3 @1 mm <- 360 x 512^2 @0.8 mm, Poisson I0=1e4, Joseph FP + voxel BP",
+            "target_rel_residual": tol, "max_iterations": max_iter, **out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +115,12 @@ def main():
     ap.add_argument("--stack-dtype", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--cpu-views", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cg", action="store_true", help="also time CG to a relative residual of --cg-tol")
+    ap.add_argument("--cg-tol", type=float, default=1e-3)
+    ap.add_argument("--cg-iter", type=int, default=100)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB (HBM counter calibration)")
+    ap.add_argument("--cg-only", action="store_true", help="skip the FDK measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -85,6 +133,17 @@ def main():
 
     rtk.lib()
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.calib_copy_mib:
+        a = torch.ones(args.calib_copy_mib * 262144, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"calibration copy of {args.calib_copy_mib} MiB done")
+        del a, b
+    if args.cg_only:
+        print(json.dumps({"cg": cg_measurement(rtk, np, torch, args.cg_iter, args.cg_tol)}), flush=True)
+        return
     log("building workload (analytic projections of the 20-ellipsoid phantom)")
     rows, g, q, proj, vol = build_workload(rtk, np)
     torch.cuda.synchronize()
@@ -136,6 +195,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(achieved / 8000.0, 4), "traffic": None},
     }
+    if args.cg:
+        result["cg"] = cg_measurement(rtk, np, torch, args.cg_iter, args.cg_tol)
     if not args.no_cpu:
         log(f"cpu baseline: oracle FDK on the first {args.cpu_views} views")
         result["cpu_baseline"] = cpu_baseline(np, rows, proj, vol, args.cpu_views)

@@ -44,10 +44,35 @@ StdMin(double a, double b)
   return (b < a) ? b : a;
 }
 
+// Copy of the volume with z fastest (index z + nz (y + ny x)), so that rays whose
+// dominant direction is x read their in-slice (y, z) taps from neighbouring
+// addresses, as z-dominant rays do from the original x-fastest layout.
+__global__ __launch_bounds__(256) void transpose_xz_kernel(const float * __restrict__ in, float * __restrict__ out,
+                                                            int nx, int ny, int nz)
+{
+  __shared__ float tile[32][33];
+  const int x0 = blockIdx.x * 32, z0 = blockIdx.y * 32, y = blockIdx.z;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5; // 32 x 8
+  for (int r = ty; r < 32; r += 8)
+  {
+    const int x = x0 + tx, z = z0 + r;
+    if (x < nx && z < nz)
+      tile[r][tx] = in[(size_t(z) * ny + y) * nx + x];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8)
+  {
+    const int z = z0 + tx, x = x0 + r;
+    if (x < nx && z < nz)
+      out[(size_t(x) * ny + y) * nz + z] = tile[tx][r];
+  }
+}
+
 template <bool COUNT>
 __global__ __launch_bounds__(256) void joseph_kernel(const JosephView * __restrict__ views, int nu, int nv,
                                                       const float * pin, float * pout,
-                                                      const float * __restrict__ vol, int nx, int ny, int nz,
+                                                      const float * __restrict__ vol,
+                                                      const float * __restrict__ volT, int nx, int ny, int nz,
                                                       double spx, double spy, double spz, double infClip,
                                                       double supClip, unsigned long long * samples)
 {
@@ -71,8 +96,11 @@ __global__ __launch_bounds__(256) void joseph_kernel(const JosephView * __restri
       mainDir = 1;
     if (fabs(dir[2]) > fabs(dir[mainDir]))
       mainDir = 2;
-    const int    n3[3] = { nx, ny, nz };
-    const long   off3[3] = { 1, long(nx), long(nx) * ny };
+    const int n3[3] = { nx, ny, nz };
+    // x-dominant rays use the z-fastest copy; offsets fit in 32 bits (checked on the host).
+    const bool  useT = (mainDir == 0);
+    const int   off3[3] = { useT ? ny * nz : 1, useT ? nz : nx, useT ? 1 : nx * ny };
+    const float * volume = useT ? volT : vol;
     const double sp3[3] = { spx, spy, spz };
     double       bmax[3];
 #pragma unroll
@@ -126,9 +154,9 @@ __global__ __launch_bounds__(256) void joseph_kernel(const JosephView * __restri
       }
       const double maxx = bmax[dInf], maxy = bmax[dSup];
       const int    nxm2 = n3[dInf] - 2, nym2 = n3[dSup] - 2;
-      const long   ox = off3[dInf], oy = off3[dSup];
-      long         oz = off3[mainDir];
-      const float * b = vol + ns * oz;
+      const int     ox = off3[dInf], oy = off3[dSup];
+      int           oz = off3[mainDir];
+      const float * b = volume + ns * oz;
       double        residualB = ns - npt[mainDir], residualE = fpt[mainDir] - fs;
       const double  norm = 1. / dir[mainDir];
       double        stepx = dir[dInf] * norm, stepy = dir[dSup] * norm;
@@ -149,10 +177,10 @@ __global__ __launch_bounds__(256) void joseph_kernel(const JosephView * __restri
       auto border = [&](double step, const float * bb, double x, double y) -> float {
         // |residual * step| <= 0.5 keeps floor in [-1, N-1]; the clamp guards rounding.
         const int    ix = min(max(int(floor(x)), -1), nxm2 + 1), iy = min(max(int(floor(y)), -1), nym2 + 1);
-        const long   idx = ix * ox + iy * oy;
+        const int    idx = ix * ox + iy * oy;
         const double lx = x - ix, ly = y - iy, lxc = 1. - lx, lyc = 1. - ly;
-        const long   xi = (ix < 0) ? ox : 0, yi = (iy < 0) ? oy : 0;
-        const long   xs = (ix >= maxx) ? -ox : 0, ys = (iy >= maxy) ? -oy : 0;
+        const int    xi = (ix < 0) ? ox : 0, yi = (iy < 0) ? oy : 0;
+        const int    xs = (ix >= maxx) ? -ox : 0, ys = (iy >= maxy) ? -oy : 0;
         float        r = 0.f;
         r += float(lxc * lyc) * bb[idx + xi + yi];
         r += float(lxc * ly) * bb[idx + oy + xi + ys];
@@ -170,20 +198,45 @@ __global__ __launch_bounds__(256) void joseph_kernel(const JosephView * __restri
         cx += stepx;
         cy += stepy;
         const int nmid = abs(fs - ns) - 1;
-        for (int t = 0; t < nmid; ++t)
+        // Middle steps, BilinearInterpolation (.hxx:443-477).  The in-slice
+        // position is kept as an integer index plus an FP64 fraction advanced
+        // by one add per step (|step| <= 1 along the dominant axis, so the index
+        // moves by at most one); the weights are formed in FP32 from the FP64
+        // fraction.  Middle samples lie on the segment inside the box; the clamp
+        // only guards grazing rays whose rounded position leaves [0, N-2].
+        int    ix = int(floor(cx)), iy = int(floor(cy));
+        double fx = cx - ix, fy = cy - iy;
+        const unsigned uox = unsigned(ox), uoy = unsigned(oy);
+        // One step: tap offset (relative to the slice) and FP32 weights, then advance.
+        auto stage = [&](int & idx, float & lx, float & ly) {
+          const int cix = min(max(ix, 0), nxm2), ciy = min(max(iy, 0), nym2);
+          idx = int(__umul24(unsigned(cix), uox) + __umul24(unsigned(ciy), uoy));
+          lx = float(fx);
+          ly = float(fy);
+          fx += stepx;
+          fy += stepy;
+          const bool xu = fx >= 1., xd = fx < 0., yu = fy >= 1., yd = fy < 0.;
+          fx = xu ? fx - 1. : (xd ? fx + 1. : fx);
+          fy = yu ? fy - 1. : (yd ? fy + 1. : fy);
+          ix += xu ? 1 : (xd ? -1 : 0);
+          iy += yu ? 1 : (yd ? -1 : 0);
+        };
+        auto tap4 = [&](const float * bb, int idx, float lx, float ly) {
+          const float lxc = 1.f - lx, lyc = 1.f - ly;
+          return (lxc * lyc) * bb[idx] + (lx * lyc) * bb[idx + ox] + (lxc * ly) * bb[idx + oy] +
+                 (lx * ly) * bb[idx + ox + oy];
+        };
+        int t = 0;
+        for (; t < nmid; ++t)
         {
-          // BilinearInterpolation (.hxx:443-477).  Middle samples lie on the
-          // segment inside the box; the clamp only guards grazing rays whose
-          // rounded position leaves [0, N-2] (the CPU would read outside).
-          const int    ix = min(max(int(floor(cx)), 0), nxm2), iy = min(max(int(floor(cy)), 0), nym2);
-          const long   idx = ix * ox + iy * oy;
-          const double lx = cx - ix, ly = cy - iy, lxc = 1. - lx, lyc = 1. - ly;
-          const float  v00 = b[idx], v10 = b[idx + ox], v01 = b[idx + oy], v11 = b[idx + ox + oy];
-          sum += float(lxc * lyc) * v00 + float(lx * lyc) * v10 + float(lxc * ly) * v01 + float(lx * ly) * v11;
+          int   id;
+          float wx, wy;
+          stage(id, wx, wy);
+          sum += tap4(b, id, wx, wy);
           b += oz;
-          cx += stepx;
-          cy += stepy;
         }
+        cx = ix + fx;
+        cy = iy + fy;
         sum += border(residualE + 0.5, b, cx, cy);
       }
       cnt = abs(fs - ns) + 1;
@@ -203,6 +256,7 @@ __global__ __launch_bounds__(256) void joseph_kernel(const JosephView * __restri
 }
 
 thread_local std::map<hipStream_t, DeviceBuffer<JosephView>>          g_views;
+thread_local std::map<hipStream_t, DeviceBuffer<float>>               g_volT;
 thread_local std::map<hipStream_t, DeviceBuffer<unsigned long long>> g_count;
 
 } // namespace
@@ -222,6 +276,9 @@ rtkhip_joseph_forward_project(const int proj_dim[3], const int vol_dim[3], const
     RTKHIP_REQUIRE(nu > 0 && nv > 0 && np >= 0 && vol_dim[0] > 0 && vol_dim[1] > 0 && vol_dim[2] > 0,
                    RTKHIP_ERR_INVALID_ARGUMENT, "joseph_forward_project: sizes must be positive");
     RTKHIP_REQUIRE(np <= 65535, RTKHIP_ERR_UNSUPPORTED, "joseph_forward_project: at most 65535 views per call");
+    const long long nvox = (long long)vol_dim[0] * vol_dim[1] * vol_dim[2];
+    RTKHIP_REQUIRE(nvox < (1LL << 31), RTKHIP_ERR_UNSUPPORTED,
+                   "joseph_forward_project: volumes are limited to 2^31 voxels (32-bit offsets)");
     RequireDevice();
     hipStream_t s = Stream(stream);
     if (samples)
@@ -240,6 +297,14 @@ rtkhip_joseph_forward_project(const int proj_dim[3], const int vol_dim[3], const
     auto & dv = g_views[s];
     dv.Resize(np);
     RTKHIP_CHECK(hipMemcpyAsync(dv.get(), hv.data(), sizeof(JosephView) * np, hipMemcpyHostToDevice, s));
+    auto & dT = g_volT[s];
+    dT.Resize(size_t(nvox));
+    {
+      dim3 tg((vol_dim[0] + 31) / 32, (vol_dim[2] + 31) / 32, vol_dim[1]);
+      hipLaunchKernelGGL(transpose_xz_kernel, tg, dim3(256), 0, s, dev_vol, dT.get(), vol_dim[0], vol_dim[1],
+                         vol_dim[2]);
+      RTKHIP_CHECK_LAUNCH();
+    }
     // Ray clip (.hxx:216-218): understood pixel->source, hence inverted.
     const double infClip = 1. - superior_clip, supClip = 1. - inferior_clip;
     dim3         grid((nu + 15) / 16, (nv + 15) / 16, np);
@@ -249,7 +314,7 @@ rtkhip_joseph_forward_project(const int proj_dim[3], const int vol_dim[3], const
       dc.Resize(1);
       RTKHIP_CHECK(hipMemsetAsync(dc.get(), 0, sizeof(unsigned long long), s));
       hipLaunchKernelGGL(joseph_kernel<true>, grid, dim3(256), 0, s, dv.get(), nu, nv, dev_proj_in, dev_proj_out,
-                         dev_vol, vol_dim[0], vol_dim[1], vol_dim[2], vol_spacing[0], vol_spacing[1],
+                         dev_vol, dT.get(), vol_dim[0], vol_dim[1], vol_dim[2], vol_spacing[0], vol_spacing[1],
                          vol_spacing[2], infClip, supClip, dc.get());
       RTKHIP_CHECK_LAUNCH();
       unsigned long long h = 0;
@@ -260,7 +325,7 @@ rtkhip_joseph_forward_project(const int proj_dim[3], const int vol_dim[3], const
     else
     {
       hipLaunchKernelGGL(joseph_kernel<false>, grid, dim3(256), 0, s, dv.get(), nu, nv, dev_proj_in, dev_proj_out,
-                         dev_vol, vol_dim[0], vol_dim[1], vol_dim[2], vol_spacing[0], vol_spacing[1],
+                         dev_vol, dT.get(), vol_dim[0], vol_dim[1], vol_dim[2], vol_spacing[0], vol_spacing[1],
                          vol_spacing[2], infClip, supClip, nullptr);
       RTKHIP_CHECK_LAUNCH();
     }
