@@ -243,6 +243,34 @@ int rtkhip_project_quadrics(rtkhip_context * c, const rtkhip_geometry * g, const
                             const float * dev_in, float * dev_out, int nshapes, const double * shapes,
                             void * stream);
 
+/* ============================================================ learned warm start */
+/* A 3D residual CNN mapping an FDK volume to a CG warm start.  No counterpart in
+ * the reference; it feeds the warm-start input of CG (RTK resumes CG from any
+ * input volume, applications/rtkconjugategradient/rtkconjugategradient.cxx:61-77).
+ * Fixed architecture: six 3x3x3 convolutions, 32 channels, ReLU after the first
+ * five, y = x + conv6(...) / scale.  BF16 weights/activations (MFMA), FP32
+ * accumulation, FP32 master parameters and Adam state.  Parameter vector layout:
+ * W1[27][32], b1[32], then for layers 2..5 W[27][32 out][32 in], b[32], then
+ * W6[27][32], b6[1]; tap t = (dz*3 + dy)*3 + dx, offset (dx-1, dy-1, dz-1). */
+typedef struct rtkhip_cnn rtkhip_cnn;
+int          rtkhip_cnn_num_params(void);
+rtkhip_cnn * rtkhip_cnn_create(void);
+void         rtkhip_cnn_destroy(rtkhip_cnn * net);
+/* Upload parameters (host, FP32) and reset the Adam moments. */
+int rtkhip_cnn_set_params(rtkhip_cnn * net, const float * host_params, void * stream);
+/* Download parameters and/or the last gradients (either pointer may be NULL). */
+int rtkhip_cnn_get_params(rtkhip_cnn * net, float * host_params, float * host_grads, void * stream);
+/* y = net(x) for `batch` volumes of dim {X, Y, Z} (x-fastest FP32, batch slowest).
+ * Keeps the activations (and a reference to dev_in) for rtkhip_cnn_backward. */
+int rtkhip_cnn_forward(rtkhip_cnn * net, int batch, const int dim[3], const float * dev_in, float * dev_out,
+                       float scale, void * stream);
+/* Loss = mean((scale (y - target))^2) of the last forward and its parameter
+ * gradient (kept in the handle); *loss (optional) receives the loss. */
+int rtkhip_cnn_backward(rtkhip_cnn * net, const float * dev_out, const float * dev_target, double * loss,
+                        void * stream);
+/* One Adam update with bias correction (step >= 1) from the last gradient. */
+int rtkhip_cnn_adam_step(rtkhip_cnn * net, float lr, float beta1, float beta2, float eps, int step, void * stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,6 +78,14 @@ SIGNATURES = [
                                   PI3, P]),
     ("rtkhip_draw_quadrics", I, [ctypes.POINTER(Image), P, P, I, PD, P]),
     ("rtkhip_project_quadrics", I, [P, P, ctypes.POINTER(Image), P, P, I, PD, P]),
+    ("rtkhip_cnn_num_params", I, []),
+    ("rtkhip_cnn_create", P, []),
+    ("rtkhip_cnn_destroy", None, [P]),
+    ("rtkhip_cnn_set_params", I, [P, P, P]),
+    ("rtkhip_cnn_get_params", I, [P, P, P, P]),
+    ("rtkhip_cnn_forward", I, [P, I, PI3, P, P, ctypes.c_float, P]),
+    ("rtkhip_cnn_backward", I, [P, P, P, PD, P]),
+    ("rtkhip_cnn_adam_step", I, [P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, I, P]),
 ]
 
 _lib = None

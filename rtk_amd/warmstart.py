@@ -1,0 +1,121 @@
+"""Learned CG warm start: a 3D residual CNN on the HIP back end.
+
+The network maps an FDK volume x to y = x + conv6(relu(...relu(conv1(s x))...)) / s
+(six 3x3x3 convolutions, 32 channels; BF16 MFMA kernels with FP32 accumulation, FP32
+master weights and Adam, rtk_amd/csrc/cnn.hip).  CG then starts from y instead of
+zero; plain CG from zero remains the fallback.  There is no counterpart in the
+reference: RTK's CG accepts any input volume as X0 (the resume path of
+applications/rtkconjugategradient/rtkconjugategradient.cxx:61-77), which is where the
+prediction enters.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib
+
+C, T = 32, 27
+
+
+def num_params():
+    return lib().rtkhip_cnn_num_params()
+
+
+def init_params(seed=0, out_scale=1e-3):
+    """He-normal initialisation, last layer small (the network starts near identity)."""
+    rng = np.random.default_rng(seed)
+    parts = [rng.normal(0, np.sqrt(2 / 27), T * C), np.zeros(C)]
+    for _ in range(4):
+        parts += [rng.normal(0, np.sqrt(2 / (27 * C)), T * C * C), np.zeros(C)]
+    parts += [rng.normal(0, out_scale, T * C), np.zeros(1)]
+    p = np.concatenate(parts).astype(np.float32)
+    assert p.size == num_params()
+    return p
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class WarmStartCNN:
+    """Inference and training of the warm-start network (device-resident)."""
+
+    def __init__(self, params=None, scale=50.0, seed=0):
+        self._h = lib().rtkhip_cnn_create()
+        if not self._h:
+            raise _lib.RtkHipError(_lib.RTKHIP_ERR_NO_DEVICE, lib().rtkhip_last_error().decode())
+        self.scale = float(scale)
+        self.step = 0
+        self.set_params(init_params(seed) if params is None else params)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib._lib is not None:
+            _lib._lib.rtkhip_cnn_destroy(h)
+            self._h = None
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(np.asarray(p, dtype=np.float32).ravel())
+        if p.size != num_params():
+            raise ValueError(f"expected {num_params()} parameters, got {p.size}")
+        check(lib().rtkhip_cnn_set_params(self._h, p.ctypes.data_as(ctypes.c_void_p), _stream()))
+        self.step = 0
+
+    def params(self):
+        p = np.empty(num_params(), np.float32)
+        check(lib().rtkhip_cnn_get_params(self._h, p.ctypes.data_as(ctypes.c_void_p), None, _stream()))
+        return p
+
+    def grads(self):
+        g = np.empty(num_params(), np.float32)
+        check(lib().rtkhip_cnn_get_params(self._h, None, g.ctypes.data_as(ctypes.c_void_p), _stream()))
+        return g
+
+    @staticmethod
+    def _batched(x):
+        if x.dim() == 3:
+            x = x[None]
+        if x.dim() != 4 or not x.is_cuda or x.dtype != torch.float32:
+            raise ValueError("expected a CUDA float32 tensor [B, Z, Y, X] or [Z, Y, X]")
+        return x.contiguous()
+
+    def forward(self, x):
+        """Network output for x ([Z, Y, X] or [B, Z, Y, X]); same shape as x."""
+        squeeze = x.dim() == 3
+        xb = self._batched(x)
+        self._x = xb  # kept alive for backward
+        y = torch.empty_like(xb)
+        dims = _lib.int3((xb.shape[3], xb.shape[2], xb.shape[1]))
+        check(lib().rtkhip_cnn_forward(self._h, xb.shape[0], dims, ctypes.c_void_p(xb.data_ptr()),
+                                       ctypes.c_void_p(y.data_ptr()), ctypes.c_float(self.scale), _stream()))
+        self._y = y
+        return y[0] if squeeze else y
+
+    __call__ = forward
+
+    def loss_and_grads(self, x, target):
+        """Forward + backward; returns (loss, gradient vector) without updating."""
+        y = self.forward(x)
+        t = self._batched(target)
+        loss = ctypes.c_double()
+        check(lib().rtkhip_cnn_backward(self._h, ctypes.c_void_p(self._y.data_ptr()), ctypes.c_void_p(t.data_ptr()),
+                                        ctypes.byref(loss), _stream()))
+        return loss.value, y
+
+    def train_step(self, x, target, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        """One Adam step on mean((scale (net(x) - target))^2); returns the loss."""
+        loss, _ = self.loss_and_grads(x, target)
+        self.step += 1
+        check(lib().rtkhip_cnn_adam_step(self._h, ctypes.c_float(lr), ctypes.c_float(beta1), ctypes.c_float(beta2),
+                                         ctypes.c_float(eps), self.step, _stream()))
+        return loss
+
+    def save(self, path):
+        np.savez(path, params=self.params(), scale=np.float64(self.scale))
+
+    @classmethod
+    def load(cls, path):
+        d = np.load(path, allow_pickle=False)
+        return cls(params=d["params"], scale=float(d["scale"]))
