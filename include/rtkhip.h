@@ -109,6 +109,17 @@ int rtkhip_joseph_forward_project(const int proj_dim[3], const int vol_dim[3], c
                                   double superior_clip, const float * dev_proj_in, float * dev_proj_out,
                                   const float * dev_vol, long long * samples, void * stream);
 
+/* Joseph backprojection: the splat transpose of rtkhip_joseph_forward_project
+ * (include/rtkJosephBackProjectionImageFilter.hxx:58-412; the backprojector of
+ * BP_JOSEPH, include/rtkIterativeConeBeamReconstructionFilter.hxx:35-103).  Same
+ * geometry arguments as the forward projector.  dev_vol_out = dev_vol_in +
+ * BP(dev_proj); dev_vol_in NULL = zeros; in == out allowed.  Accumulates with
+ * FP32 atomics (results reproducible to rounding, not bitwise). */
+int rtkhip_joseph_backproject(const int proj_dim[3], const int vol_dim[3], const double * proj_to_vol,
+                              const double * source, const double vol_spacing[3], double inferior_clip,
+                              double superior_clip, const float * dev_vol_in, float * dev_vol_out,
+                              const float * dev_proj, void * stream);
+
 /* FDK cosine/angular weighting.  Replaces CUDA_weight_projection
  * (src/rtkCudaFDKWeightProjectionFilter.cu:86-129) with the CPU filter's math
  * (include/rtkFDKWeightProjectionFilter.hxx:67-128).
@@ -218,8 +229,21 @@ int rtkhip_back_project(rtkhip_context * c, const rtkhip_geometry * g, const rtk
                         const float * dev_vol_in, float * dev_vol_out, const rtkhip_image * stack,
                         const void * dev_proj, int proj_dtype, int fdk, void * stream);
 
-/* rtk::HipConjugateGradientConeBeamReconstructionFilter with Joseph FP and voxel BP
- * (include/rtkConjugateGradientConeBeamReconstructionFilter.hxx:195-305):
+/* Joseph backprojector at filter level (rtk::JosephBackProjectionImageFilter).
+ * vol_out = vol_in + BP_Joseph(proj). */
+int rtkhip_joseph_back_project(rtkhip_context * c, const rtkhip_geometry * g, const rtkhip_image * vol,
+                               const float * dev_vol_in, float * dev_vol_out, const rtkhip_image * stack,
+                               const float * dev_proj, void * stream);
+
+/* Backprojector of the CG operator (RTK's BackProjectionType subset). */
+enum rtkhip_backprojector
+{
+  RTKHIP_BP_VOXELBASED = 0, /* RTK's default: A is not symmetric */
+  RTKHIP_BP_JOSEPH = 1      /* matched adjoint of the Joseph FP: A symmetric */
+};
+
+/* rtk::HipConjugateGradientConeBeamReconstructionFilter with the Joseph FP and the
+ * chosen backprojector (include/rtkConjugateGradientConeBeamReconstructionFilter.hxx:195-305):
  * solves BP(W FP(x)) = BP(W p) from dev_x (X0 on entry, result on exit).
  *   dev_weights  optional projection weights (NULL = ones)
  *   niter        maximum iterations
@@ -229,7 +253,7 @@ int rtkhip_back_project(rtkhip_context * c, const rtkhip_geometry * g, const rtk
  *   iterations   optional: iterations done */
 int rtkhip_cg_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const rtkhip_image * vol, float * dev_x,
                           const rtkhip_image * stack, const float * dev_proj, const float * dev_weights, int niter,
-                          double rel_tol, double * resid, int * iterations, void * stream);
+                          double rel_tol, int backprojector, double * resid, int * iterations, void * stream);
 
 /* Phantom generators (input generation for tests and benchmarks).  Each shape is
  * 11 doubles: quadric coefficients A..J of A x^2 + B y^2 + C z^2 + D xy + E xz +

@@ -1089,6 +1089,168 @@ JosephForward(const Geometry & g, const ImageInfo & si, const T * projIn, T * pr
   return total;
 }
 
+// ------------------------------------------------------------------ Joseph BP --
+// include/rtkJosephBackProjectionImageFilter.hxx:58-412 with the default functors
+// (ValueAlongRay returns the pixel value; SplatWeightMultiplication adds
+// value * weight * |stepMM| * stepLength, .h:110-136): the splat transpose of the
+// Joseph forward projector.  vol is updated in place.  RTK runs it single-threaded;
+// here views are split over threads, each with a private accumulator, summed in
+// thread order.
+template <class T>
+void
+JosephBack(const Geometry & g, const ImageInfo & vi, T * vol, const ImageInfo & si, const T * proj, int nthreads,
+           int geomOffset = 0, double inferiorClipIn = 0., double superiorClipIn = 1.)
+{
+  if (g.radius != 0.)
+    throw std::runtime_error("cylindrical detector not supported by the oracle");
+  const int np = si.size[2], nu = si.size[0], nv = si.size[1];
+  if ((int)g.size() < geomOffset + np)
+    throw std::runtime_error("Mismatch between the number of projections and the geometry entries.");
+  const long offsets[3] = { 1, vi.size[0], long(vi.size[0]) * vi.size[1] };
+  const M44  volPPToIndex = PhysicalToIndex(vi);
+  double     boxMin[3], boxMax[3];
+  for (int i = 0; i < 3; i++)
+  {
+    boxMin[i] = 0.;
+    boxMax[i] = (vi.size[i] - 1) * (1. - std::numeric_limits<double>::epsilon());
+  }
+  const double inferiorClip = 1. - superiorClipIn, superiorClip = 1. - inferiorClipIn;
+  nthreads = std::max(1, std::min(nthreads, np));
+  std::vector<std::vector<T>> acc(nthreads, std::vector<T>(vi.N(), T(0)));
+  std::vector<std::thread>    th;
+  for (int tid = 0; tid < nthreads; tid++)
+    th.emplace_back([&, tid]() {
+      T * out = acc[tid].data();
+      for (int p = tid; p < np; p += nthreads)
+      {
+        const int gi = p + geomOffset;
+        double    s4[4], src[3];
+        g.SourcePosition(gi, s4);
+        for (int r = 0; r < 3; r++)
+        {
+          src[r] = 0.;
+          for (int c = 0; c < 4; c++)
+            src[r] += volPPToIndex.a[r][c] * s4[c];
+        }
+        const M44 pm = volPPToIndex * (g.ProjectionCoordinatesToFixedSystem(gi) * IndexToPhysical(si));
+        for (int jv = 0; jv < nv; jv++)
+          for (int iu = 0; iu < nu; iu++)
+          {
+            const T rayValue = proj[size_t(p) * nu * nv + size_t(jv) * nu + iu];
+            double  pos[3], dir[3];
+            for (int r = 0; r < 3; r++)
+            {
+              pos[r] = pm.a[r][3] + pm.a[r][0] * iu + pm.a[r][1] * jv + pm.a[r][2] * p;
+              dir[r] = -(pos[r] - src[r]);
+            }
+            int mainDir = 0;
+            for (int i = 0; i < 3; i++)
+              if (std::abs(dir[i]) > std::abs(dir[mainDir]))
+                mainDir = i;
+            double infDist = -std::numeric_limits<double>::max(), supDist = std::numeric_limits<double>::max();
+            bool   hit = true;
+            for (int i = 0; i < 3 && hit; i++)
+            {
+              if (dir[i] == 0. && (pos[i] < boxMin[i] || pos[i] > boxMax[i]))
+              {
+                hit = false;
+                break;
+              }
+              double inv = 1 / dir[i];
+              double T1 = (boxMin[i] - pos[i]) * inv, T2 = (boxMax[i] - pos[i]) * inv;
+              if (T1 > T2)
+                std::swap(T1, T2);
+              infDist = std::max(T1, infDist);
+              supDist = std::min(T2, supDist);
+              if (infDist > supDist || supDist < 0.)
+                hit = false;
+            }
+            // .hxx:148-155: test before clipping
+            if (!(hit && supDist >= 0. && infDist <= 1.))
+              continue;
+            infDist = std::max(infDist, inferiorClip);
+            supDist = std::min(supDist, superiorClip);
+            double npnt[3], fpnt[3];
+            for (int r = 0; r < 3; r++)
+            {
+              npnt[r] = pos[r] + infDist * dir[r];
+              fpnt[r] = pos[r] + supDist * dir[r];
+            }
+            const int ns = (int)std::lround(npnt[mainDir]), fs = (int)std::lround(fpnt[mainDir]);
+            int       dInf = (mainDir + 1) % 3, dSup = (mainDir + 2) % 3;
+            if (dInf > dSup)
+              std::swap(dInf, dSup);
+            const double maxx = boxMax[dInf], maxy = boxMax[dSup];
+            const long   ox = offsets[dInf], oy = offsets[dSup];
+            long         oz = offsets[mainDir];
+            T *          pxiyi = out + ns * oz;
+            double       residualB = ns - npnt[mainDir], residualE = fpnt[mainDir] - fs;
+            const double norm = 1. / dir[mainDir];
+            double       stepx = dir[dInf] * norm, stepy = dir[dSup] * norm;
+            if (npnt[mainDir] > fpnt[mainDir])
+            {
+              residualB *= -1;
+              residualE *= -1;
+              oz *= -1;
+              stepx *= -1;
+              stepy *= -1;
+            }
+            double currentx = npnt[dInf] + residualB * stepx;
+            double currenty = npnt[dSup] + residualB * stepy;
+            double stepMM[3];
+            stepMM[dInf] = vi.spacing[dInf] * stepx;
+            stepMM[dSup] = vi.spacing[dSup] * stepy;
+            stepMM[mainDir] = vi.spacing[mainDir];
+            const double voxelSize = std::sqrt(stepMM[0] * stepMM[0] + stepMM[1] * stepMM[1] + stepMM[2] * stepMM[2]);
+            // BilinearSplat (.hxx:296-318) and BilinearSplatOnBorders (.hxx:326-370)
+            auto splat = [&](T * b, double step, double x, double y, bool borders) {
+              int    ix = (int)std::floor(x), iy = (int)std::floor(y);
+              long   idx = ix * ox + iy * oy;
+              double lx = x - ix, ly = y - iy, lxc = 1. - lx, lyc = 1. - ly;
+              long   xi = 0, yi = 0, xs = 0, ys = 0;
+              if (borders)
+              {
+                if (ix < 0)
+                  xi = ox;
+                if (iy < 0)
+                  yi = oy;
+                if (ix >= maxx)
+                  xs = -ox;
+                if (iy >= maxy)
+                  ys = -oy;
+              }
+              b[idx + xi + yi] += rayValue * (lxc * lyc) * voxelSize * step;
+              b[idx + oy + xi + ys] += rayValue * (lxc * ly) * voxelSize * step;
+              b[idx + ox + xs + yi] += rayValue * (lx * lyc) * voxelSize * step;
+              b[idx + ox + oy + xs + ys] += rayValue * (lx * ly) * voxelSize * step;
+            };
+            if (fs == ns)
+              splat(pxiyi, std::abs(fpnt[mainDir] - npnt[mainDir]), currentx, currenty, true);
+            else
+            {
+              splat(pxiyi, residualB + 0.5, currentx, currenty, true);
+              pxiyi += oz;
+              currentx += stepx;
+              currenty += stepy;
+              for (int i = 0; i < std::abs(fs - ns) - 1; ++i)
+              {
+                splat(pxiyi, 1.0, currentx, currenty, false);
+                pxiyi += oz;
+                currentx += stepx;
+                currenty += stepy;
+              }
+              splat(pxiyi, residualE + 0.5, currentx, currenty, true);
+            }
+          }
+      }
+    });
+  for (auto & x : th)
+    x.join();
+  for (int t = 0; t < nthreads; t++)
+    for (size_t i = 0; i < vi.N(); i++)
+      vol[i] += acc[t][i];
+}
+
 // ------------------------------------------------------------------------ CG --
 // ConjugateGradientConeBeamReconstructionFilter (include/rtkConjugateGradientConeBeamReconstructionFilter.hxx:195-305)
 // with A x = BP_voxel(0, W * FP_joseph(0, x)) (include/rtkReconstructionConjugateGradientOperator.hxx:194-286)
@@ -1099,13 +1261,20 @@ JosephForward(const Geometry & g, const ImageInfo & si, const T * projIn, T * pr
 template <class T>
 void
 ConjugateGradient(const Geometry & g, const ImageInfo & vi, T * x, const ImageInfo & si, const T * proj,
-                  const T * weights, int niter, int nthreads, double * resid = nullptr)
+                  const T * weights, int niter, int nthreads, double * resid = nullptr, bool josephBP = false)
 {
   const size_t   nv = vi.N(), ns = si.N();
   std::vector<T> pw(ns), tmp(ns), B(nv, 0), AX(nv), R(nv), P(nv);
   for (size_t i = 0; i < ns; i++)
     pw[i] = weights ? T(proj[i] * weights[i]) : proj[i];
-  Backproject(g, vi, B.data(), si, pw.data(), false, nthreads);
+  // BP_VOXELBASED (default) or BP_JOSEPH (include/rtkIterativeConeBeamReconstructionFilter.hxx:35-103)
+  auto bp = [&](T * out, const T * p) {
+    if (josephBP)
+      JosephBack(g, vi, out, si, p, nthreads);
+    else
+      Backproject(g, vi, out, si, p, false, nthreads);
+  };
+  bp(B.data(), pw.data());
   auto applyA = [&](const T * in, T * out) {
     std::vector<T> zp(ns, 0);
     JosephForward(g, si, zp.data(), tmp.data(), vi, in, nthreads);
@@ -1113,7 +1282,7 @@ ConjugateGradient(const Geometry & g, const ImageInfo & vi, T * x, const ImageIn
       for (size_t i = 0; i < ns; i++)
         tmp[i] = T(tmp[i] * weights[i]);
     std::fill(out, out + nv, T(0));
-    Backproject(g, vi, out, si, tmp.data(), false, nthreads);
+    bp(out, tmp.data());
   };
   applyA(x, AX.data());
   for (size_t i = 0; i < nv; i++)

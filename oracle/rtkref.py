@@ -142,6 +142,16 @@ def joseph(g9, stack, proj_in, vol, volume, threads=8, geom_offset=0, return_sam
     return (out, cnt.value) if return_samples else out
 
 
+def joseph_back(g9, vol, volume, stack, proj, threads=8, geom_offset=0):
+    """Returns volume + JosephBP(proj) (the splat transpose of the Joseph FP)."""
+    n, g = _geom(g9)
+    v = np.array(volume, copy=True, order="C")
+    proj = np.ascontiguousarray(proj, dtype=v.dtype)
+    fn = getattr(lib(), "rtkref_joseph_back_" + _suffix(v.dtype))
+    _check(fn(n, _ptr(g), ctypes.byref(vol), _ptr(v), ctypes.byref(stack), _ptr(proj), threads, geom_offset))
+    return v
+
+
 def fdk(g9, vol, volume, stack, proj, threads=8, subset=16, **ramp):
     n, g = _geom(g9)
     v = np.array(volume, copy=True, order="C")
@@ -152,8 +162,9 @@ def fdk(g9, vol, volume, stack, proj, threads=8, subset=16, **ramp):
     return v
 
 
-def conjugate_gradient(g9, vol, x0, stack, proj, niter, weights=None, threads=8):
-    """Returns (x, residual_norm2[niter+1]) of plain CG (scalars in the pixel type)."""
+def conjugate_gradient(g9, vol, x0, stack, proj, niter, weights=None, threads=8, joseph_bp=False):
+    """Returns (x, residual_norm2[niter+1]) of plain CG (scalars in the pixel type);
+    joseph_bp selects BP_JOSEPH instead of BP_VOXELBASED."""
     n, g = _geom(g9)
     x = np.array(x0, copy=True, order="C")
     proj = np.ascontiguousarray(proj, dtype=x.dtype)
@@ -161,7 +172,7 @@ def conjugate_gradient(g9, vol, x0, stack, proj, niter, weights=None, threads=8)
     res = np.zeros(niter + 1)
     fn = getattr(lib(), "rtkref_cg_" + _suffix(x.dtype))
     _check(fn(n, _ptr(g), ctypes.byref(vol), _ptr(x), ctypes.byref(stack), _ptr(proj), _ptr(w), niter, threads,
-              _ptr(res)))
+              _ptr(res), int(joseph_bp)))
     return x, res
 
 

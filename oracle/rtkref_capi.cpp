@@ -197,6 +197,12 @@ extern "C"
         *samples = c;                                                                                               \
     });                                                                                                             \
   }                                                                                                                 \
+  int rtkref_joseph_back_##S(int n, const double * g9, const rtkref_image * vol, T * volume,                     \
+                             const rtkref_image * stack, const T * proj, int threads, int geomOffset)               \
+  {                                                                                                                 \
+    return Guard(                                                                                                   \
+      [&] { JosephBack<T>(MakeGeometry(n, g9), Info(vol), volume, Info(stack), proj, threads, geomOffset); });      \
+  }                                                                                                                 \
   int rtkref_fdk_##S(int n, const double * g9, const rtkref_image * vol, T * volume, const rtkref_image * stack,   \
                      const T * proj, const double * opts, int threads, int subset)                                  \
   {                                                                                                                 \
@@ -205,10 +211,11 @@ extern "C"
     });                                                                                                             \
   }                                                                                                                 \
   int rtkref_cg_##S(int n, const double * g9, const rtkref_image * vol, T * x, const rtkref_image * stack,         \
-                    const T * proj, const T * weights, int niter, int threads, double * resid)                      \
+                    const T * proj, const T * weights, int niter, int threads, double * resid, int josephBP)        \
   {                                                                                                                 \
     return Guard([&] {                                                                                              \
-      ConjugateGradient<T>(MakeGeometry(n, g9), Info(vol), x, Info(stack), proj, weights, niter, threads, resid);   \
+      ConjugateGradient<T>(                                                                                         \
+        MakeGeometry(n, g9), Info(vol), x, Info(stack), proj, weights, niter, threads, resid, josephBP != 0);       \
     });                                                                                                             \
   }                                                                                                                 \
   int rtkref_project_quadrics_##S(int n, const double * g9, const rtkref_image * stack, const T * in, T * out,     \

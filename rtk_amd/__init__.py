@@ -14,6 +14,7 @@ from .filters import (  # noqa: F401
     HipFDKBackProjectionImageFilter,
     HipFDKConeBeamReconstructionFilter,
     HipForwardProjectionImageFilter,
+    HipJosephBackProjectionImageFilter,
     Image,
     RampFilterOptions,
     ThreeDCircularProjectionGeometry,

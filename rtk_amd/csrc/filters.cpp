@@ -812,12 +812,36 @@ rtkhip_fdk_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const rtkh
 }
 
 extern "C" int
+rtkhip_joseph_back_project(rtkhip_context * c, const rtkhip_geometry * g, const rtkhip_image * vol,
+                           const float * dev_vol_in, float * dev_vol_out, const rtkhip_image * stack,
+                           const float * dev_proj, void * stream)
+{
+  return Guard([&] {
+    RTKHIP_REQUIRE(c, RTKHIP_ERR_INVALID_ARGUMENT, "joseph_back_project: null context");
+    CheckImage(vol, "joseph_back_project volume");
+    CheckImage(stack, "joseph_back_project projections");
+    CheckGeometry(g, *stack, true);
+    RTKHIP_REQUIRE(g->views[0].sdd != 0., RTKHIP_ERR_UNSUPPORTED,
+                   "parallel geometry is not supported by the HIP Joseph projector");
+    std::vector<double> p2v, src;
+    JosephRays(*g, *stack, *vol, p2v, src);
+    const int pd[3] = { stack->size[0], stack->size[1], stack->size[2] };
+    const int vd[3] = { vol->size[0], vol->size[1], vol->size[2] };
+    Throw(rtkhip_joseph_backproject(pd, vd, p2v.data(), src.data(), vol->spacing, 0., 1., dev_vol_in, dev_vol_out,
+                                    dev_proj, stream));
+    RTKHIP_CHECK(hipStreamSynchronize(Stream(stream)));
+  });
+}
+
+extern "C" int
 rtkhip_cg_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const rtkhip_image * vol, float * dev_x,
                       const rtkhip_image * stack, const float * dev_proj, const float * dev_weights, int niter,
-                      double rel_tol, double * resid, int * iterations, void * stream)
+                      double rel_tol, int backprojector, double * resid, int * iterations, void * stream)
 {
   return Guard([&] {
     RTKHIP_REQUIRE(c && dev_x && dev_proj && niter >= 0, RTKHIP_ERR_INVALID_ARGUMENT, "cg_reconstruct: bad argument");
+    RTKHIP_REQUIRE(backprojector == RTKHIP_BP_VOXELBASED || backprojector == RTKHIP_BP_JOSEPH,
+                   RTKHIP_ERR_INVALID_ARGUMENT, "cg_reconstruct: unknown backprojector");
     CheckImage(vol, "cg volume");
     CheckImage(stack, "cg projections");
     CheckGeometry(g, *stack, true);
@@ -841,7 +865,12 @@ rtkhip_cg_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const rtkhi
     const int vd[3] = { vol->size[0], vol->size[1], vol->size[2] };
     float *   T = c->pT.get();
     auto      bp = [&](const float * proj, float * out) {
-      Throw(rtkhip_voxel_backproject(pd, vd, mats.data(), border.data(), nullptr, out, proj, RTKHIP_F32, stream));
+      if (backprojector == RTKHIP_BP_JOSEPH)
+        Throw(rtkhip_joseph_backproject(pd, vd, p2v.data(), src.data(), vol->spacing, 0., 1., nullptr, out, proj,
+                                        stream));
+      else
+        Throw(
+          rtkhip_voxel_backproject(pd, vd, mats.data(), border.data(), nullptr, out, proj, RTKHIP_F32, stream));
     };
     // A x = BP(0, W FP(0, x))   (ReconstructionConjugateGradientOperator)
     auto applyA = [&](const float * x, float * out) {

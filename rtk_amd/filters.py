@@ -325,6 +325,23 @@ class HipBackProjectionImageFilter(_Filter):
         self._output = vol.like(out)
 
 
+class HipJosephBackProjectionImageFilter(_Filter):
+    """Joseph backprojector (splat transpose of the Joseph FP): input 0 = volume to
+    add to, input 1 = projections (include/rtkJosephBackProjectionImageFilter.h)."""
+
+    def Update(self):
+        vol, proj = self._inputs[0], self._inputs[1]
+        _require_cuda(vol, "BP volume")
+        _require_cuda(proj, "BP projections")
+        if proj.tensor.dtype != torch.float32:
+            raise ValueError("the Joseph backprojector reads FP32 projections")
+        out = self._out_for(vol)
+        vd, pd = vol.descriptor(), proj.descriptor()
+        check(lib().rtkhip_joseph_back_project(_context(), self._geom(), ctypes.byref(vd), _ptr(vol.tensor),
+                                               _ptr(out), ctypes.byref(pd), _ptr(proj.tensor), _stream()))
+        self._output = vol.like(out)
+
+
 class HipFDKBackProjectionImageFilter(HipBackProjectionImageFilter):
     """rtk::FDKBackProjectionImageFilter on HIP (w^2-weighted, normalised at the centre)."""
 
@@ -336,8 +353,11 @@ class HipConjugateGradientConeBeamReconstructionFilter(_Filter):
     input 0 = X0, input 1 = projections; optional weights
     (include/rtkConjugateGradientConeBeamReconstructionFilter.hxx:195-305)."""
 
+    BP_VOXELBASED, BP_JOSEPH = _lib.BP_VOXELBASED, _lib.BP_JOSEPH
+
     def __init__(self):
         super().__init__()
+        self._bp = _lib.BP_VOXELBASED  # reference default (rtkIterativeConeBeamReconstructionFilter.hxx:29-30)
         self._niter = 3  # reference default (.hxx:36)
         self._tol = 0.
         self._weights = None
@@ -346,6 +366,12 @@ class HipConjugateGradientConeBeamReconstructionFilter(_Filter):
 
     def SetNumberOfIterations(self, n):
         self._niter = int(n)
+
+    def SetBackProjectionFilter(self, bp):
+        """BP_VOXELBASED (RTK's default) or BP_JOSEPH (matched adjoint)."""
+        if bp not in (_lib.BP_VOXELBASED, _lib.BP_JOSEPH):
+            raise ValueError("unsupported backprojector")
+        self._bp = int(bp)
 
     def SetInputWeights(self, w):
         self._weights = w
@@ -370,7 +396,7 @@ class HipConjugateGradientConeBeamReconstructionFilter(_Filter):
         vd, pd = vol.descriptor(), proj.descriptor()
         w = self._weights.tensor if self._weights is not None else None
         check(lib().rtkhip_cg_reconstruct(_context(), self._geom(), ctypes.byref(vd), _ptr(x), ctypes.byref(pd),
-                                          _ptr(proj.tensor), _ptr(w), self._niter, self._tol,
+                                          _ptr(proj.tensor), _ptr(w), self._niter, self._tol, self._bp,
                                           res.ctypes.data_as(_lib.PD), ctypes.byref(it), _stream()))
         self._iters = it.value
         self._resid = res[: it.value + 1]

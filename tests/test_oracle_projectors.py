@@ -138,3 +138,20 @@ def test_joseph_float_close_to_double(oracle):
     pd = oracle.joseph(rows, st, np.zeros((8, 26, 30)), vol, v)
     pf = oracle.joseph(rows, st, np.zeros((8, 26, 30), np.float32), vol, v.astype(np.float32))
     assert np.linalg.norm(pf - pd) / np.linalg.norm(pd) < 1e-6
+
+
+def test_joseph_backprojector_is_adjoint(oracle):
+    """test/rtkadjointoperatorstest.cxx:118-147 in double: <F v, p> = <v, B p> (the
+    reference asserts 1e-3; the restatement is exact to rounding)."""
+    for rows in (np.array([[600., 1000., 22.5 * i + 1, 2.5 * i - 4, -3 + 0.5 * i, 0, 0, 5 - 0.3 * i, 2.]
+                           for i in range(16)]),
+                 np.array([[30., 1000., 22.5 * i, 0, 0, 3., 2., 0, 0] for i in range(16)])):
+        vol = oracle.image((22, 24, 20), (-30, -33, -28), (3, 2.8, 3))
+        st = oracle.image((40, 32, 16), (-70, -55, 0), (3.5, 3.5, 1))
+        rng = np.random.default_rng(0)
+        v = rng.random((20, 24, 22))
+        p = rng.random((16, 32, 40))
+        fv = oracle.joseph(rows, st, np.zeros_like(p), vol, v)
+        bp = oracle.joseph_back(rows, vol, np.zeros_like(v), st, p)
+        a, b = (fv * p).sum(), (v * bp).sum()
+        assert abs(a / b - 1) < 1e-12
