@@ -29,6 +29,7 @@
 #include <hip/amd_detail/amd_hip_unsafe_atomics.h>
 
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <map>
 #include <vector>
@@ -108,6 +109,7 @@ struct Ray
   int    start;          // offset of slice ns (ns times the unsigned slice offset)
   int    nxm2, nym2;     // in-slice index limits (N - 2)
   bool   transposed;     // dominant axis x: z-fastest buffer
+  int    axis;           // dominant axis (0 x, 1 y, 2 z)
   double cx, cy;         // in-slice position at slice ns
   double stepx, stepy;   // in-slice advance per slice
   double maxx, maxy;     // box limits used by the border clamp
@@ -199,6 +201,7 @@ SetupRay(const JosephView & V, int iu, int jv, int p, int nx, int ny, int nz, do
     dSup = t;
   }
   R.transposed = useT;
+  R.axis = mainDir;
   R.maxx = bmax[dInf];
   R.maxy = bmax[dSup];
   R.nxm2 = n3[dInf] - 2;
@@ -229,25 +232,40 @@ SetupRay(const JosephView & V, int iu, int jv, int p, int nx, int ny, int nz, do
   return true;
 }
 
-// Tap offsets and FP32 weights of a first/last step (BilinearInterpolationOnBorders,
-// .hxx:493-543: taps outside the box are clamped inward).
+// In-slice tap coordinates and FP32 weights of a first/last step
+// (BilinearInterpolationOnBorders, .hxx:493-543: taps outside the box are clamped
+// inward).
 __device__ __forceinline__ void
-BorderTaps(const Ray & R, double x, double y, int t[4], float w[4])
+BorderTaps2D(const Ray & R, double x, double y, int tx[4], int ty[4], float w[4])
 {
   // |residual * step| <= 0.5 keeps floor in [-1, N-1]; the clamp guards rounding.
   const int    ix = min(max(int(floor(x)), -1), R.nxm2 + 1), iy = min(max(int(floor(y)), -1), R.nym2 + 1);
-  const int    idx = ix * R.ox + iy * R.oy;
   const double lx = x - ix, ly = y - iy, lxc = 1. - lx, lyc = 1. - ly;
-  const int    xi = (ix < 0) ? R.ox : 0, yi = (iy < 0) ? R.oy : 0;
-  const int    xs = (ix >= R.maxx) ? -R.ox : 0, ys = (iy >= R.maxy) ? -R.oy : 0;
-  t[0] = idx + xi + yi;
+  const int    x0 = ix + ((ix < 0) ? 1 : 0), y0 = iy + ((iy < 0) ? 1 : 0);
+  const int    x1 = ix + 1 - ((ix >= R.maxx) ? 1 : 0), y1 = iy + 1 - ((iy >= R.maxy) ? 1 : 0);
+  tx[0] = x0;
+  ty[0] = y0;
   w[0] = float(lxc * lyc);
-  t[1] = idx + R.oy + xi + ys;
+  tx[1] = x0;
+  ty[1] = y1;
   w[1] = float(lxc * ly);
-  t[2] = idx + R.ox + xs + yi;
+  tx[2] = x1;
+  ty[2] = y0;
   w[2] = float(lx * lyc);
-  t[3] = idx + R.ox + R.oy + xs + ys;
+  tx[3] = x1;
+  ty[3] = y1;
   w[3] = float(lx * ly);
+}
+
+// Same taps as buffer offsets relative to the slice.
+__device__ __forceinline__ void
+BorderTaps(const Ray & R, double x, double y, int t[4], float w[4])
+{
+  int tx[4], ty[4];
+  BorderTaps2D(R, x, y, tx, ty, w);
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    t[q] = tx[q] * R.ox + ty[q] * R.oy;
 }
 
 // Middle steps: the in-slice position as integer index + FP64 fraction.  Middle
@@ -266,8 +284,15 @@ struct Walker
   }
   __device__ __forceinline__ int Stage(const Ray & R, float & lx, float & ly)
   {
-    const int cix = min(max(ix, 0), R.nxm2), ciy = min(max(iy, 0), R.nym2);
-    const int idx = int(__umul24(unsigned(cix), unsigned(R.ox)) + __umul24(unsigned(ciy), unsigned(R.oy)));
+    int cix, ciy;
+    Stage2D(R, cix, ciy, lx, ly);
+    return int(__umul24(unsigned(cix), unsigned(R.ox)) + __umul24(unsigned(ciy), unsigned(R.oy)));
+  }
+  // Lower-left tap (cix, ciy) of the current sample, then advance one slice.
+  __device__ __forceinline__ void Stage2D(const Ray & R, int & cix, int & ciy, float & lx, float & ly)
+  {
+    cix = min(max(ix, 0), R.nxm2);
+    ciy = min(max(iy, 0), R.nym2);
     lx = float(fx);
     ly = float(fy);
     fx += R.stepx;
@@ -277,7 +302,6 @@ struct Walker
     fy = yu ? fy - 1. : (yd ? fy + 1. : fy);
     ix += xu ? 1 : (xd ? -1 : 0);
     iy += yu ? 1 : (yd ? -1 : 0);
-    return idx;
   }
 };
 
@@ -348,21 +372,11 @@ __global__ __launch_bounds__(256) void joseph_kernel(const JosephView * __restri
 }
 
 // Splat transpose: every sample adds value * weight * |stepMM| * stepLength to its
-// four taps (BilinearSplat / BilinearSplatOnBorders, .hxx:296-370).
-__global__ __launch_bounds__(256) void joseph_bp_kernel(const JosephView * __restrict__ views, int nu, int nv,
-                                                         const float * __restrict__ proj, float * acc,
-                                                         float * accT, int nx, int ny, int nz, double spx,
-                                                         double spy, double spz, double infClip, double supClip)
+// four taps (BilinearSplat / BilinearSplatOnBorders, .hxx:296-370).  One ray,
+// straight to the accumulator with FP32 atomics.
+__device__ __forceinline__ void
+SplatRay(const Ray & R, float value, float * acc, float * accT)
 {
-  const int iu = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int jv = blockIdx.y * 16 + (threadIdx.x >> 4);
-  const int p = blockIdx.z;
-  if (iu >= nu || jv >= nv)
-    return;
-  const float value = proj[(long(p) * nv + jv) * nu + iu];
-  Ray         R;
-  if (value == 0.f || !SetupRay<true>(views[p], iu, jv, p, nx, ny, nz, spx, spy, spz, infClip, supClip, R))
-    return;
   float *      b = (R.transposed ? accT : acc) + R.start;
   const double vn = double(value) * R.stepNorm;
   auto         border = [&](double step, float * bb, double x, double y) {
@@ -396,6 +410,262 @@ __global__ __launch_bounds__(256) void joseph_bp_kernel(const JosephView * __res
     b += R.oz;
   }
   border(R.wE, b, wk.ix + wk.fx, wk.iy + wk.fy);
+}
+
+// Tiled splat.  A workgroup is a 16 x 16 tile of detector pixels of one view.  Rays
+// of the tile that share the tile's most common dominant axis and walking direction
+// step through the slices in lockstep; each slice's taps are summed in an LDS window
+// and the window is then added to the accumulator with one global FP32 atomic per
+// touched voxel, instead of one per ray tap: a 16 x 16 bundle covers about 10 x 10
+// voxels of a slice, so global atomic traffic drops by about 10x.  The window holds
+// 64-bit fixed point (ds_add_u64; LDS float atomics run at a fraction of the integer
+// rate on gfx950): each tap's FP32 contribution, computed exactly as the per-ray path
+// computes it, is scaled by a power of two chosen from the tile's largest |value| *
+// |stepMM| (an upper bound of every contribution, with room for 2^11 of them per
+// voxel), so the window sums are exact to 2^-50 of that bound and are rounded to FP32
+// once, at the flush.  Rows are padded to 33 cells to spread LDS banks.  The window
+// is double-buffered so a slice is flushed while the next one is accumulated, with
+// one barrier per slice.  The window of slice k is placed from the four corner rays
+// of the tile: a central projection maps the pixel rectangle onto a convex
+// quadrilateral of the slice plane, so the corners bound every ray's position; its
+// width is a convex function of k, so checking the first and last slice bounds all.
+// Taps that still fall outside (rounding), rays of a minority axis or direction, and
+// tiles whose footprint exceeds the window or that hold a non-finite value take the
+// per-ray path.
+constexpr int kTileWin = 32;    // LDS window edge, in voxels
+constexpr int kTileStride = 33; // LDS row stride, in cells
+
+__global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * __restrict__ views, int nu, int nv,
+                                                              const float * __restrict__ proj, float * acc,
+                                                              float * accT, int nx, int ny, int nz, double spx,
+                                                              double spy, double spz, double infClip,
+                                                              double supClip)
+{
+  __shared__ unsigned long long win[2][kTileWin * kTileStride];
+  __shared__ int    vote[6];
+  __shared__ int    maxBits; // largest |value * stepNorm| of the participating rays, as FP32 bits
+  __shared__ int    krange[2]; // min and max slice over the tile's participating rays
+  __shared__ double line[4][4]; // corner c: in-slice x = line[c][0] + k line[c][1], y likewise
+  __shared__ int    cornerOk;
+
+  const int tid = threadIdx.x;
+  const int u0 = blockIdx.x * 16, v0 = blockIdx.y * 16, p = blockIdx.z;
+  const int iu = u0 + (tid & 15), jv = v0 + (tid >> 4);
+  for (int c = tid; c < 2 * kTileWin * kTileStride; c += 256)
+    (&win[0][0])[c] = 0ull;
+  if (tid < 6)
+    vote[tid] = 0;
+  if (tid == 0)
+  {
+    krange[0] = INT_MAX;
+    krange[1] = INT_MIN;
+    cornerOk = 1;
+    maxBits = 0;
+  }
+  __syncthreads();
+
+  const JosephView & V = views[p];
+  Ray                R;
+  float              value = 0.f;
+  bool               active = false;
+  if (iu < nu && jv < nv)
+  {
+    value = proj[(long(p) * nv + jv) * nu + iu];
+    active = value != 0.f && SetupRay<true>(V, iu, jv, p, nx, ny, nz, spx, spy, spz, infClip, supClip, R);
+  }
+  const int key = active ? 2 * R.axis + (R.oz < 0 ? 1 : 0) : -1;
+  if (active)
+    atomicAdd(&vote[key], 1);
+  __syncthreads();
+  int best = 0;
+  for (int q = 1; q < 6; q++)
+    best = vote[q] > vote[best] ? q : best;
+  const int  axis = best >> 1, dirSign = (best & 1) ? -1 : 1;
+  const bool part = active && key == best;
+  if (part)
+  {
+    atomicMin(&krange[0], min(R.ns, R.fs));
+    atomicMax(&krange[1], max(R.ns, R.fs));
+    // non-negative FP32 bit patterns order like the values; NaN and Inf sort above
+    atomicMax(&maxBits, __float_as_int(fabsf(float(double(value) * R.stepNorm))));
+  }
+  int dInf = (axis + 1) % 3, dSup = (axis + 2) % 3;
+  if (dInf > dSup)
+  {
+    const int t = dInf;
+    dInf = dSup;
+    dSup = t;
+  }
+  if (tid < 4)
+  {
+    // corner pixels of the part of the tile inside the detector
+    const int    cu = (tid & 1) ? min(u0 + 15, nu - 1) : u0, cv = (tid & 2) ? min(v0 + 15, nv - 1) : v0;
+    double       pos[3], dir[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+    {
+      pos[r] = V.m[4 * r + 3] + V.m[4 * r + 0] * cu + V.m[4 * r + 1] * cv + V.m[4 * r + 2] * p;
+      dir[r] = V.src[r] - pos[r];
+    }
+    const double dm = dir[axis];
+    if (!(dm * dirSign > 0.)) // walking direction is the sign of the dominant component
+      cornerOk = 0;
+    else
+    {
+      const double bx = dir[dInf] / dm, by = dir[dSup] / dm;
+      line[tid][0] = pos[dInf] - pos[axis] * bx;
+      line[tid][1] = bx;
+      line[tid][2] = pos[dSup] - pos[axis] * by;
+      line[tid][3] = by;
+    }
+  }
+  __syncthreads();
+  const int   kmin = krange[0], kmax = krange[1];
+  const float maxc = __int_as_float(maxBits);
+  bool        tiled = cornerOk && kmin <= kmax && maxc <= FLT_MAX;
+  auto      window = [&](int k, int & wx0, int & wy0, double & spanx, double & spany) {
+    double lox = DBL_MAX, hix = -DBL_MAX, loy = DBL_MAX, hiy = -DBL_MAX;
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+    {
+      const double x = line[c][0] + k * line[c][1], y = line[c][2] + k * line[c][3];
+      lox = fmin(lox, x);
+      hix = fmax(hix, x);
+      loy = fmin(loy, y);
+      hiy = fmax(hiy, y);
+    }
+    wx0 = int(floor(lox)) - 1;
+    wy0 = int(floor(loy)) - 1;
+    spanx = hix - lox;
+    spany = hiy - loy;
+  };
+  if (tiled)
+  {
+    int    wx, wy;
+    double sx0, sy0, sx1, sy1;
+    window(kmin, wx, wy, sx0, sy0);
+    window(kmax, wx, wy, sx1, sy1);
+    // taps reach floor(hi) + 1, the window starts at floor(lo) - 1: span + 4 cells
+    tiled = fmax(sx0, sx1) + 5. <= kTileWin && fmax(sy0, sy1) + 5. <= kTileWin;
+  }
+  if (!tiled)
+  {
+    if (active)
+      SplatRay(R, value, acc, accT);
+    return; // uniform across the workgroup
+  }
+  if (active && !part)
+    SplatRay(R, value, acc, accT);
+
+  // Lockstep walk over the tile's slices in the walking direction.
+  const int    nsl = kmax - kmin + 1;
+  const int    kfirst = dirSign > 0 ? kmin : kmax;
+  const bool   useT = axis == 0;
+  float *      buf = useT ? accT : acc;
+  const int    n3[3] = { nx, ny, nz };
+  const int    off3[3] = { useT ? ny * nz : 1, useT ? nz : nx, useT ? 1 : nx * ny };
+  const int    ox = off3[dInf], oy = off3[dSup], oz = off3[axis];
+  const int    NX = n3[dInf], NY = n3[dSup];
+  const int    r0 = part ? (R.ns - kfirst) * dirSign : INT_MAX; // this ray's first and last tile step
+  const int    r1 = part ? (R.fs - kfirst) * dirSign : INT_MIN;
+  const double vn = part ? double(value) * R.stepNorm : 0.;
+  // every contribution is at most maxc * (1 + rounding): 2^11 of them stay below 2^61
+  int          emax;
+  frexp(double(maxc), &emax);
+  const double toFixed = ldexp(1., 50 - emax), fromFixed = ldexp(1., emax - 50);
+  Walker       wk;
+  int          pwx = 0, pwy = 0; // window origin of the previous slice
+  for (int s = 0; s <= nsl; ++s)
+  {
+    const int k = kfirst + s * dirSign;
+    int       wx = 0, wy = 0;
+    if (s < nsl)
+    {
+      double sx, sy;
+      window(k, wx, wy, sx, sy);
+      if (s >= r0 && s <= r1)
+      {
+        unsigned long long * w = win[s & 1];
+        float *              g = buf + size_t(k) * oz;
+        int     tx[4], ty[4];
+        float   tw[4];
+        float   sc;
+        const int t = s - r0;
+        if (t == 0 || t > R.nmid)
+        {
+          double step, x, y;
+          if (t == 0)
+          {
+            step = (R.fs == R.ns) ? R.wOne : R.wB;
+            x = R.cx;
+            y = R.cy;
+            wk.Init(R.cx + R.stepx, R.cy + R.stepy);
+          }
+          else
+          {
+            step = R.wE;
+            x = wk.ix + wk.fx;
+            y = wk.iy + wk.fy;
+          }
+          BorderTaps2D(R, x, y, tx, ty, tw);
+          sc = float(vn * step);
+        }
+        else
+        {
+          int   cx, cy;
+          float lx, ly;
+          wk.Stage2D(R, cx, cy, lx, ly);
+          const float lxc = 1.f - lx, lyc = 1.f - ly;
+          tx[0] = cx;
+          ty[0] = cy;
+          tw[0] = lxc * lyc;
+          tx[1] = cx;
+          ty[1] = cy + 1;
+          tw[1] = lxc * ly;
+          tx[2] = cx + 1;
+          ty[2] = cy;
+          tw[2] = lx * lyc;
+          tx[3] = cx + 1;
+          ty[3] = cy + 1;
+          tw[3] = lx * ly;
+          sc = float(vn);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+        {
+          const int   a = tx[q] - wx, b = ty[q] - wy;
+          const float v = sc * tw[q];
+          if (unsigned(a) < unsigned(kTileWin) && unsigned(b) < unsigned(kTileWin))
+            atomicAdd(&w[b * kTileStride + a], (unsigned long long)__double2ll_rn(double(v) * toFixed));
+          else
+            unsafeAtomicAdd(g + tx[q] * ox + ty[q] * oy, v);
+        }
+      }
+    }
+    if (s > 0)
+    {
+      // flush slice s - 1; lanes run along the unit-stride in-slice axis
+      const int k1 = k - dirSign;
+      unsigned long long * w = win[(s - 1) & 1];
+      float *              g = buf + size_t(k1) * oz;
+      for (int c = tid; c < kTileWin * kTileWin; c += 256)
+      {
+        const int                a = c & (kTileWin - 1), b = c / kTileWin;
+        const int                i = (ox == 1) ? a : b, j = (ox == 1) ? b : a;
+        const unsigned long long q = w[j * kTileStride + i];
+        if (q != 0ull)
+        {
+          w[j * kTileStride + i] = 0ull;
+          const int X = pwx + i, Y = pwy + j;
+          if (X >= 0 && X < NX && Y >= 0 && Y < NY)
+            unsafeAtomicAdd(g + X * ox + Y * oy, float(double((long long)q) * fromFixed));
+        }
+      }
+    }
+    pwx = wx;
+    pwy = wy;
+    __syncthreads();
+  }
 }
 
 struct JosephCache
@@ -523,7 +793,7 @@ rtkhip_joseph_backproject(const int proj_dim[3], const int vol_dim[3], const dou
     RTKHIP_CHECK(hipMemsetAsync(dT.get(), 0, nvox * sizeof(float), s));
     const double infClip = 1. - superior_clip, supClip = 1. - inferior_clip;
     dim3         grid((nu + 15) / 16, (nv + 15) / 16, np);
-    hipLaunchKernelGGL(joseph_bp_kernel, grid, dim3(256), 0, s, dv, nu, nv, dev_proj, dev_vol_out, dT.get(),
+    hipLaunchKernelGGL(joseph_bp_tile_kernel, grid, dim3(256), 0, s, dv, nu, nv, dev_proj, dev_vol_out, dT.get(),
                        vol_dim[0], vol_dim[1], vol_dim[2], vol_spacing[0], vol_spacing[1], vol_spacing[2], infClip,
                        supClip);
     RTKHIP_CHECK_LAUNCH();

@@ -8,19 +8,28 @@ from helpers import circular_rows, hip_geometry, rel_l2, to_device
 
 pytestmark = pytest.mark.gpu
 
+SMALL = dict(vsize=(37, 33, 29), vsp=(3., 2.5, 3.5), vorig=(-53., -40., -49.), nu=61, nv=47,
+             sorig=(-96., -73., 0.), ssp=(3.2, 3.2, 1.))
+# Full 16 x 16 detector tiles at the clinical shape: views on the axes, on the
+# diagonals (tiles split between x- and z-dominant rays) and in between.
+FULL = dict(vsize=(128, 128, 128), vsp=(1., 1., 1.), vorig=(-63.5,) * 3, nu=256, nv=256,
+            sorig=(-102., -102., 0.), ssp=(0.8, 0.8, 1.))
 GEOMS = {
-    "offsets": np.array([[600., 1000., 22.5 * i + 1, 2.5 * i - 4, -3. + 0.5 * i, 0, 0, 5. - 0.3 * i, 2.]
-                         for i in range(16)]),
-    "inner_tilted": np.array([[30., 1000., 22.5 * i, 0, 0, 3., 2., 0, 0] for i in range(16)]),
+    "offsets": (np.array([[600., 1000., 22.5 * i + 1, 2.5 * i - 4, -3. + 0.5 * i, 0, 0, 5. - 0.3 * i, 2.]
+                          for i in range(16)]), SMALL),
+    "inner_tilted": (np.array([[30., 1000., 22.5 * i, 0, 0, 3., 2., 0, 0] for i in range(16)]), SMALL),
+    "clinical_tiles": (np.array([[1000., 1536., a, 0, 0, 0, 0, 0, 0]
+                                 for a in (0., 45., 90., 135., 180., 225., 270., 315., 17., 62., 113., 331.)]),
+                       FULL),
 }
 
 
 @pytest.mark.parametrize("gname", list(GEOMS))
 def test_joseph_bp_parity_and_adjoint(oracle, hip, gname):
-    rows = GEOMS[gname]
-    vsize, vsp, vorig = (37, 33, 29), (3., 2.5, 3.5), (-53., -40., -49.)
-    nu, nv, n = 61, 47, len(rows)
-    sorig, ssp = (-96., -73., 0.), (3.2, 3.2, 1.)
+    rows, sz = GEOMS[gname]
+    vsize, vsp, vorig = sz["vsize"], sz["vsp"], sz["vorig"]
+    nu, nv, n = sz["nu"], sz["nv"], len(rows)
+    sorig, ssp = sz["sorig"], sz["ssp"]
     rng = np.random.default_rng(21)
     p = rng.normal(size=(n, nv, nu)).astype(np.float32)
     v0 = rng.normal(size=vsize[::-1]).astype(np.float32)
