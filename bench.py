@@ -14,6 +14,13 @@ algorithmic 16 bytes per voxel update (four FP32 taps), timed with HIP events on
 the stream it runs on.  cpu_baseline = the oracle's multithreaded FDK (a
 line-by-line restatement of RTK's CPU filters) on a bounded sample of the same
 workload (all voxels, first views only), timed on this host.
+
+The second half of the metric is reported beside it (skip with --no-cg):
+"cg" times plain CG at 256^3 from 360 noisy views of 512^2 (30 iterations with RTK's
+default Joseph FP + voxel BP pair, then time to a relative residual of 1e-3 with
+that pair and with the matched Joseph pair); "cg_warm_start" times CG to 1e-3 at
+256^3 from 90 noisy views of 512^2 on held-out phantom seeds, from zero, from FDK and
+from the committed warm-start network's CNN(FDK), start costs included.
 """
 import argparse
 import json
@@ -26,6 +33,10 @@ sys.path.insert(0, ROOT)
 
 NX, NU, NVIEWS = 512, 1024, 720
 SEED = 20260001
+# HBM bytes per launch of the FDK backprojection kernel on this workload (FP32 stack):
+# rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB device copy),
+# profiles/bp_fdk_512_720_pmc.md.  Re-measure when that kernel changes.
+BP_TRAFFIC_BYTES = 1.863e11
 
 
 def log(msg):
@@ -66,8 +77,10 @@ def cpu_baseline(np, rows, proj, vol, views):
 
 
 def cg_measurement(rtk, np, torch, max_iter, tol):
-    """Plain CG (Joseph FP + voxel BP, FP64 scalars) on 256^3 @1 mm from 360 views of
-    512^2 @0.8 mm with Poisson noise (I0 = 1e4), from zero and from the FDK volume."""
+    """Plain CG on 256^3 @1 mm from 360 views of 512^2 @0.8 mm with Poisson noise
+    (I0 = 1e4): 30 iterations from zero with RTK's default pair (Joseph FP + voxel
+    BP, FP64 scalars), then time to a relative residual `tol` from zero with that
+    pair and with the matched Joseph pair."""
     n, nu, nx = 360, 512, 256
     g = rtk.ThreeDCircularProjectionGeometry()
     for i in range(n):
@@ -81,30 +94,61 @@ def cg_measurement(rtk, np, torch, max_iter, tol):
     proj = clean.like((-torch.log(torch.clamp(counts, min=1.) / 1e4)).float().contiguous())
     vol = rtk.ConstantImageSource((nx, nx, nx), (1., 1., 1.), (-(nx - 1) / 2.,) * 3)
     out = {}
-    fdk = rtk.HipFDKConeBeamReconstructionFilter()
-    fdk.SetInput(0, vol)
-    fdk.SetInput(1, proj)
-    fdk.SetGeometry(g)
-    fdk.Update()
-    for name, x0 in [("zero", vol), ("fdk", fdk.GetOutput())]:
+    runs = [("fixed_30_voxel_bp", 30, 0., None), ("to_tol_voxel_bp", max_iter, tol, None),
+            ("to_tol_joseph_bp", max_iter, tol, "joseph")]
+    for name, niter, t, bp in runs:
         cg = rtk.HipConjugateGradientConeBeamReconstructionFilter()
-        cg.SetInput(0, x0)
+        cg.SetInput(0, vol)
         cg.SetInput(1, proj)
         cg.SetGeometry(g)
-        cg.SetNumberOfIterations(max_iter)
-        cg.SetResidualTolerance(tol)
+        if bp == "joseph":
+            cg.SetBackProjectionFilter(cg.BP_JOSEPH)
+        cg.SetNumberOfIterations(niter)
+        cg.SetResidualTolerance(t)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         cg.Update()
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         res = cg.GetResiduals()
-        out[name] = {"seconds": round(dt, 3), "iterations": int(cg.GetNumberOfIterationsDone()),
-                     "final_rel_residual": float(res[-1]), "reached": bool(res[-1] <= tol),
-                     "ms_per_iteration": round(dt * 1e3 / max(1, cg.GetNumberOfIterationsDone()), 2)}
-        log(f"cg from {name}: {out[name]}")
-    return {"workload": "CG 256^3 @1 mm <- 360 x 512^2 @0.8 mm, Poisson I0=1e4, Joseph FP + voxel BP",
+        it = int(cg.GetNumberOfIterationsDone())
+        out[name] = {"seconds": round(dt, 3), "iterations": it, "final_rel_residual": float(res[-1]),
+                     "ms_per_iteration": round(dt * 1e3 / max(1, it), 2)}
+        if t:
+            out[name]["reached"] = bool(res[-1] <= t)
+        log(f"cg {name}: {out[name]}")
+    return {"workload": "CG 256^3 @1 mm <- 360 x 512^2 @0.8 mm, Poisson I0=1e4, zero start",
             "target_rel_residual": tol, "max_iterations": max_iter, **out}
+
+
+def warm_start_measurement(rtk, np, torch, seeds, tol):
+    """Learned warm start (sparse view): 256^3 @1 mm from 90 views of 512^2, Poisson
+    I0 = 1e4, held-out phantom seeds.  CG (Joseph pair, FP64 scalars) to a relative
+    residual `tol` from zero, from FDK and from CNN(FDK) with the committed network;
+    each start's own cost is included in its time."""
+    from rtk_amd import warmstart_train as W
+    from rtk_amd.warmstart import WarmStartCNN, default_weights_path
+    net = WarmStartCNN.load(default_weights_path())
+    g = W.geometry()
+    rows = []
+    for seed in seeds:
+        rows.append(W.evaluate_seed(net, g, seed, tol=tol))
+        log(f"warm start seed {seed}: " + ", ".join(f"{k} {rows[-1][k]['iterations']} it "
+                                                   f"{rows[-1][k]['seconds']:.2f} s" for k in ("zero", "fdk", "cnn")))
+    med = {k: float(np.median([r[k]["seconds"] for r in rows])) for k in ("zero", "fdk", "cnn")}
+    return {"workload": "CG 256^3 @1 mm <- 90 x 512^2 @0.8 mm, Poisson I0=1e4, held-out seeds",
+            "target_rel_residual": tol, "seeds": [int(s) for s in seeds],
+            "median_seconds": {k: round(v, 3) for k, v in med.items()},
+            "median_iterations": {k: float(np.median([r[k]["iterations"] for r in rows]))
+                                  for k in ("zero", "fdk", "cnn")},
+            "speedup_cnn_vs_zero": round(med["zero"] / med["cnn"], 3),
+            "speedup_fdk_vs_zero": round(med["zero"] / med["fdk"], 3),
+            "per_seed": rows}
+
+
+def _warm_seeds(args):
+    from rtk_amd import warmstart_train as W
+    return [W.EVAL_SEED0 + i for i in range(args.warm_seeds)]
 
 
 def main():
@@ -115,9 +159,10 @@ def main():
     ap.add_argument("--stack-dtype", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--cpu-views", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cg", action="store_true", help="also time CG to a relative residual of --cg-tol")
+    ap.add_argument("--no-cg", action="store_true", help="skip the CG and warm-start measurements")
     ap.add_argument("--cg-tol", type=float, default=1e-3)
     ap.add_argument("--cg-iter", type=int, default=100)
+    ap.add_argument("--warm-seeds", type=int, default=2, help="held-out seeds for the warm-start measurement")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--cg-only", action="store_true", help="skip the FDK measurement")
@@ -142,7 +187,9 @@ def main():
         log(f"calibration copy of {args.calib_copy_mib} MiB done")
         del a, b
     if args.cg_only:
-        print(json.dumps({"cg": cg_measurement(rtk, np, torch, args.cg_iter, args.cg_tol)}), flush=True)
+        print(json.dumps({"cg": cg_measurement(rtk, np, torch, args.cg_iter, args.cg_tol),
+                          "cg_warm_start": warm_start_measurement(rtk, np, torch, _warm_seeds(args), args.cg_tol)}),
+              flush=True)
         return
     log("building workload (analytic projections of the 20-ellipsoid phantom)")
     rows, g, q, proj, vol = build_workload(rtk, np)
@@ -176,7 +223,7 @@ def main():
     bytes_per_update = 16 if args.stack_dtype == "f32" else 8
     achieved = updates * bytes_per_update / (bp_avg * 1e-3) / 1e9
     result = {
-        "metric": "FDK GUPS at 512^3 (weighting + FFT ramp + voxel backprojection, 720 views of 1024^2)",
+        "metric": "FDK GUPS at 512\u00b3; CG time-to-residual 1e-3, learned warm start vs plain",
         "value": round(updates * args.steps / dt / 1e9, 3),
         "unit": "GUPS",
         "n_gpus": 1,
@@ -193,10 +240,12 @@ def main():
                    "bp_kernel_gups": round(updates / (bp_avg * 1e-3) / 1e9, 3),
                    "bp_kernel_ms": round(bp_avg, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-                     "frac": round(achieved / 8000.0, 4), "traffic": None},
+                     "frac": round(achieved / 8000.0, 4),
+                     "traffic": BP_TRAFFIC_BYTES if args.stack_dtype == "f32" else None},
     }
-    if args.cg:
+    if not args.no_cg:
         result["cg"] = cg_measurement(rtk, np, torch, args.cg_iter, args.cg_tol)
+        result["cg_warm_start"] = warm_start_measurement(rtk, np, torch, _warm_seeds(args), args.cg_tol)
     if not args.no_cpu:
         log(f"cpu baseline: oracle FDK on the first {args.cpu_views} views")
         result["cpu_baseline"] = cpu_baseline(np, rows, proj, vol, args.cpu_views)

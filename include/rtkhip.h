@@ -114,11 +114,12 @@ int rtkhip_joseph_forward_project(const int proj_dim[3], const int vol_dim[3], c
  * BP_JOSEPH, include/rtkIterativeConeBeamReconstructionFilter.hxx:35-103).  Same
  * geometry arguments as the forward projector.  dev_vol_out = dev_vol_in +
  * BP(dev_proj); dev_vol_in NULL = zeros; in == out allowed.  Accumulates with
- * FP32 atomics (results reproducible to rounding, not bitwise). */
+ * FP32 atomics (results reproducible to rounding, not bitwise).
+ *   proj_dtype enum rtkhip_dtype of dev_proj (FP32 math and accumulation in every case). */
 int rtkhip_joseph_backproject(const int proj_dim[3], const int vol_dim[3], const double * proj_to_vol,
                               const double * source, const double vol_spacing[3], double inferior_clip,
                               double superior_clip, const float * dev_vol_in, float * dev_vol_out,
-                              const float * dev_proj, void * stream);
+                              const void * dev_proj, int proj_dtype, void * stream);
 
 /* FDK cosine/angular weighting.  Replaces CUDA_weight_projection
  * (src/rtkCudaFDKWeightProjectionFilter.cu:86-129) with the CPU filter's math
@@ -230,10 +231,10 @@ int rtkhip_back_project(rtkhip_context * c, const rtkhip_geometry * g, const rtk
                         const void * dev_proj, int proj_dtype, int fdk, void * stream);
 
 /* Joseph backprojector at filter level (rtk::JosephBackProjectionImageFilter).
- * vol_out = vol_in + BP_Joseph(proj). */
+ * vol_out = vol_in + BP_Joseph(proj); proj_dtype as for rtkhip_back_project. */
 int rtkhip_joseph_back_project(rtkhip_context * c, const rtkhip_geometry * g, const rtkhip_image * vol,
                                const float * dev_vol_in, float * dev_vol_out, const rtkhip_image * stack,
-                               const float * dev_proj, void * stream);
+                               const void * dev_proj, int proj_dtype, void * stream);
 
 /* Backprojector of the CG operator (RTK's BackProjectionType subset). */
 enum rtkhip_backprojector
@@ -249,11 +250,15 @@ enum rtkhip_backprojector
  *   niter        maximum iterations
  *   rel_tol      stop once ||B - A X_k|| / ||B|| <= rel_tol (0 = run niter); the
  *                norm is the CG recursion's R_k, FP64
+ *   stack_dtype  storage of the stacks the backprojector reads (enum rtkhip_dtype): the
+ *                weighted measured projections and each iteration's W FP(P) are
+ *                rounded to FP16/BF16 once, all arithmetic stays FP32/FP64
  *   resid        optional host array of niter+1 doubles: ||R_k|| / ||B||
  *   iterations   optional: iterations done */
 int rtkhip_cg_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const rtkhip_image * vol, float * dev_x,
                           const rtkhip_image * stack, const float * dev_proj, const float * dev_weights, int niter,
-                          double rel_tol, int backprojector, double * resid, int * iterations, void * stream);
+                          double rel_tol, int backprojector, int stack_dtype, double * resid, int * iterations,
+                          void * stream);
 
 /* Phantom generators (input generation for tests and benchmarks).  Each shape is
  * 11 doubles: quadric coefficients A..J of A x^2 + B y^2 + C z^2 + D xy + E xz +

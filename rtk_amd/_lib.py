@@ -57,7 +57,7 @@ SIGNATURES = [
     ("rtkhip_voxel_backproject", I, [PI3, PI3, PD, PI3, P, P, P, I, P]),
     ("rtkhip_joseph_forward_project", I, [PI3, PI3, PD, PD, PD, D, D, P, P, P,
                                           ctypes.POINTER(ctypes.c_longlong), P]),
-    ("rtkhip_joseph_backproject", I, [PI3, PI3, PD, PD, PD, D, D, P, P, P, P]),
+    ("rtkhip_joseph_backproject", I, [PI3, PI3, PD, PD, PD, D, D, P, P, P, I, P]),
     ("rtkhip_fdk_weight", I, [PI3, PD, PD, PD, P, P, P]),
     ("rtkhip_ramp_filter", I, [PI3, I, PD, I, PD, PD, PD, PD, P, P, I, P]),
     ("rtkhip_cg_step", I, [ctypes.c_size_t, P, P, P, P, P, P]),
@@ -76,8 +76,8 @@ SIGNATURES = [
     ("rtkhip_forward_project", I, [P, P, ctypes.POINTER(Image), P, P, ctypes.POINTER(Image), P,
                                    ctypes.POINTER(ctypes.c_longlong), P]),
     ("rtkhip_back_project", I, [P, P, ctypes.POINTER(Image), P, P, ctypes.POINTER(Image), P, I, I, P]),
-    ("rtkhip_joseph_back_project", I, [P, P, ctypes.POINTER(Image), P, P, ctypes.POINTER(Image), P, P]),
-    ("rtkhip_cg_reconstruct", I, [P, P, ctypes.POINTER(Image), P, ctypes.POINTER(Image), P, P, I, D, I, PD,
+    ("rtkhip_joseph_back_project", I, [P, P, ctypes.POINTER(Image), P, P, ctypes.POINTER(Image), P, I, P]),
+    ("rtkhip_cg_reconstruct", I, [P, P, ctypes.POINTER(Image), P, ctypes.POINTER(Image), P, P, I, D, I, I, PD,
                                   PI3, P]),
     ("rtkhip_draw_quadrics", I, [ctypes.POINTER(Image), P, P, I, PD, P]),
     ("rtkhip_project_quadrics", I, [P, P, ctypes.POINTER(Image), P, P, I, PD, P]),

@@ -33,22 +33,6 @@ namespace rtkhip
 namespace
 {
 
-__device__ __forceinline__ float
-LoadPix(const float * p, long i)
-{
-  return p[i];
-}
-__device__ __forceinline__ float
-LoadPix(const __half * p, long i)
-{
-  return __half2float(p[i]);
-}
-__device__ __forceinline__ float
-LoadPix(const __hip_bfloat16 * p, long i)
-{
-  return __bfloat162float(p[i]);
-}
-
 // Per-view coefficients, centred on the volume: with x = i - ci, y = j - cj,
 // z = k - ck:  U = c[0] + c[1] x + c[2] y + c[3] z, V = c[4..7], W = c[8..11].
 struct alignas(16) ViewCoef

@@ -2,6 +2,8 @@
 // buffers.  Nothing here throws across the C ABI.
 #pragma once
 
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -13,6 +15,23 @@
 
 namespace rtkhip
 {
+
+// FP32 value of a stored projection pixel (FP32, FP16 or BF16 stacks).
+__device__ __forceinline__ float
+LoadPix(const float * p, long i)
+{
+  return p[i];
+}
+__device__ __forceinline__ float
+LoadPix(const __half * p, long i)
+{
+  return __half2float(p[i]);
+}
+__device__ __forceinline__ float
+LoadPix(const __hip_bfloat16 * p, long i)
+{
+  return __bfloat162float(p[i]);
+}
 
 // Exception carrying an rtkhip_status; converted to a return code at the ABI.
 struct Error : std::runtime_error

@@ -26,6 +26,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace rtkhip
@@ -280,6 +281,7 @@ struct rtkhip_context
 {
   rtkhip::DeviceBuffer<unsigned char> filtered;
   rtkhip::DeviceBuffer<float>         vB, vR, vP, vAP, pT;
+  rtkhip::DeviceBuffer<unsigned short> pT16; // 16-bit stack read by the CG backprojector
   rtkhip::DeviceBuffer<double>        state;
   struct RampKey
   {
@@ -542,6 +544,33 @@ Multiply(size_t n, const float * a, const float * b, float * out, hipStream_t s)
 {
   const unsigned g = unsigned(std::min<size_t>((n + 255) / 256, 4096));
   hipLaunchKernelGGL(multiply_kernel, dim3(g ? g : 1), dim3(256), 0, s, n, a, b, out);
+  RTKHIP_CHECK_LAUNCH();
+}
+
+// out = TO(a * w) (w NULL = ones): the weighted stack in its 16-bit storage type.
+template <typename TO>
+__global__ __launch_bounds__(256) void weight_store_kernel(size_t n, const float * a, const float * w, TO * out)
+{
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+  {
+    const float v = w ? a[i] * w[i] : a[i];
+    if constexpr (std::is_same<TO, __half>::value)
+      out[i] = __float2half(v);
+    else
+      out[i] = __float2bfloat16(v);
+  }
+}
+
+void
+WeightAndStore(size_t n, const float * a, const float * w, void * out, int dtype, hipStream_t s)
+{
+  const unsigned g = unsigned(std::min<size_t>((n + 255) / 256, 4096));
+  if (dtype == RTKHIP_F16)
+    hipLaunchKernelGGL(weight_store_kernel<__half>, dim3(g ? g : 1), dim3(256), 0, s, n, a, w,
+                       static_cast<__half *>(out));
+  else
+    hipLaunchKernelGGL(weight_store_kernel<__hip_bfloat16>, dim3(g ? g : 1), dim3(256), 0, s, n, a, w,
+                       static_cast<__hip_bfloat16 *>(out));
   RTKHIP_CHECK_LAUNCH();
 }
 
@@ -814,7 +843,7 @@ rtkhip_fdk_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const rtkh
 extern "C" int
 rtkhip_joseph_back_project(rtkhip_context * c, const rtkhip_geometry * g, const rtkhip_image * vol,
                            const float * dev_vol_in, float * dev_vol_out, const rtkhip_image * stack,
-                           const float * dev_proj, void * stream)
+                           const void * dev_proj, int proj_dtype, void * stream)
 {
   return Guard([&] {
     RTKHIP_REQUIRE(c, RTKHIP_ERR_INVALID_ARGUMENT, "joseph_back_project: null context");
@@ -828,7 +857,7 @@ rtkhip_joseph_back_project(rtkhip_context * c, const rtkhip_geometry * g, const 
     const int pd[3] = { stack->size[0], stack->size[1], stack->size[2] };
     const int vd[3] = { vol->size[0], vol->size[1], vol->size[2] };
     Throw(rtkhip_joseph_backproject(pd, vd, p2v.data(), src.data(), vol->spacing, 0., 1., dev_vol_in, dev_vol_out,
-                                    dev_proj, stream));
+                                    dev_proj, proj_dtype, stream));
     RTKHIP_CHECK(hipStreamSynchronize(Stream(stream)));
   });
 }
@@ -836,12 +865,15 @@ rtkhip_joseph_back_project(rtkhip_context * c, const rtkhip_geometry * g, const 
 extern "C" int
 rtkhip_cg_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const rtkhip_image * vol, float * dev_x,
                       const rtkhip_image * stack, const float * dev_proj, const float * dev_weights, int niter,
-                      double rel_tol, int backprojector, double * resid, int * iterations, void * stream)
+                      double rel_tol, int backprojector, int stack_dtype, double * resid, int * iterations,
+                      void * stream)
 {
   return Guard([&] {
     RTKHIP_REQUIRE(c && dev_x && dev_proj && niter >= 0, RTKHIP_ERR_INVALID_ARGUMENT, "cg_reconstruct: bad argument");
     RTKHIP_REQUIRE(backprojector == RTKHIP_BP_VOXELBASED || backprojector == RTKHIP_BP_JOSEPH,
                    RTKHIP_ERR_INVALID_ARGUMENT, "cg_reconstruct: unknown backprojector");
+    RTKHIP_REQUIRE(stack_dtype == RTKHIP_F32 || stack_dtype == RTKHIP_F16 || stack_dtype == RTKHIP_BF16,
+                   RTKHIP_ERR_INVALID_ARGUMENT, "cg_reconstruct: unknown stack dtype");
     CheckImage(vol, "cg volume");
     CheckImage(stack, "cg projections");
     CheckGeometry(g, *stack, true);
@@ -856,6 +888,8 @@ rtkhip_cg_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const rtkhi
     c->vP.Resize(nvox);
     c->vAP.Resize(nvox);
     c->pT.Resize(npix);
+    if (stack_dtype != RTKHIP_F32)
+      c->pT16.Resize(npix);
     c->state.Resize(8);
     std::vector<double> mats, p2v, src;
     std::vector<int>    border;
@@ -864,30 +898,33 @@ rtkhip_cg_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const rtkhi
     const int pd[3] = { stack->size[0], stack->size[1], stack->size[2] };
     const int vd[3] = { vol->size[0], vol->size[1], vol->size[2] };
     float *   T = c->pT.get();
-    auto      bp = [&](const float * proj, float * out) {
+    // BP of W q, with q rounded to the storage type first when the stack is 16-bit
+    auto bpw = [&](const float * q, float * out) {
+      const void * proj = q;
+      if (stack_dtype != RTKHIP_F32)
+      {
+        WeightAndStore(npix, q, dev_weights, c->pT16.get(), stack_dtype, s);
+        proj = c->pT16.get();
+      }
+      else if (dev_weights)
+      {
+        Multiply(npix, q, dev_weights, T, s);
+        proj = T;
+      }
       if (backprojector == RTKHIP_BP_JOSEPH)
         Throw(rtkhip_joseph_backproject(pd, vd, p2v.data(), src.data(), vol->spacing, 0., 1., nullptr, out, proj,
-                                        stream));
+                                        stack_dtype, stream));
       else
-        Throw(
-          rtkhip_voxel_backproject(pd, vd, mats.data(), border.data(), nullptr, out, proj, RTKHIP_F32, stream));
+        Throw(rtkhip_voxel_backproject(pd, vd, mats.data(), border.data(), nullptr, out, proj, stack_dtype, stream));
     };
     // A x = BP(0, W FP(0, x))   (ReconstructionConjugateGradientOperator)
     auto applyA = [&](const float * x, float * out) {
       Throw(rtkhip_joseph_forward_project(pd, vd, p2v.data(), src.data(), vol->spacing, 0., 1., nullptr, T, x, nullptr,
                                           stream));
-      if (dev_weights)
-        Multiply(npix, T, dev_weights, T, s);
-      bp(T, out);
+      bpw(T, out);
     };
     // B = BP(0, W p)
-    if (dev_weights)
-    {
-      Multiply(npix, dev_proj, dev_weights, T, s);
-      bp(T, c->vB.get());
-    }
-    else
-      bp(dev_proj, c->vB.get());
+    bpw(dev_proj, c->vB.get());
     double * st = c->state.get();
     Throw(rtkhip_dot(nvox, c->vB.get(), c->vB.get(), st + 4, stream));
     applyA(dev_x, c->vAP.get());

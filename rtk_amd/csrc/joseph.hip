@@ -435,8 +435,9 @@ SplatRay(const Ray & R, float value, float * acc, float * accT)
 constexpr int kTileWin = 32;    // LDS window edge, in voxels
 constexpr int kTileStride = 33; // LDS row stride, in cells
 
+template <typename TP>
 __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * __restrict__ views, int nu, int nv,
-                                                              const float * __restrict__ proj, float * acc,
+                                                              const TP * __restrict__ proj, float * acc,
                                                               float * accT, int nx, int ny, int nz, double spx,
                                                               double spy, double spz, double infClip,
                                                               double supClip)
@@ -470,7 +471,7 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
   bool               active = false;
   if (iu < nu && jv < nv)
   {
-    value = proj[(long(p) * nv + jv) * nu + iu];
+    value = LoadPix(proj, (long(p) * nv + jv) * nu + iu);
     active = value != 0.f && SetupRay<true>(V, iu, jv, p, nx, ny, nz, spx, spy, spz, infClip, supClip, R);
   }
   const int key = active ? 2 * R.axis + (R.oz < 0 ? 1 : 0) : -1;
@@ -767,13 +768,15 @@ rtkhip_joseph_forward_project(const int proj_dim[3], const int vol_dim[3], const
 extern "C" int
 rtkhip_joseph_backproject(const int proj_dim[3], const int vol_dim[3], const double * proj_to_vol,
                           const double * source, const double vol_spacing[3], double inferior_clip,
-                          double superior_clip, const float * dev_vol_in, float * dev_vol_out, const float * dev_proj,
-                          void * stream)
+                          double superior_clip, const float * dev_vol_in, float * dev_vol_out, const void * dev_proj,
+                          int proj_dtype, void * stream)
 {
   using namespace rtkhip;
   return Guard([&] {
     CheckArgs(proj_dim, vol_dim, proj_to_vol, source, vol_spacing);
     RTKHIP_REQUIRE(dev_vol_out && dev_proj, RTKHIP_ERR_INVALID_ARGUMENT, "joseph_backproject: null buffer");
+    RTKHIP_REQUIRE(proj_dtype == RTKHIP_F32 || proj_dtype == RTKHIP_F16 || proj_dtype == RTKHIP_BF16,
+                   RTKHIP_ERR_INVALID_ARGUMENT, "joseph_backproject: unknown projection dtype");
     RequireDevice();
     hipStream_t  s = Stream(stream);
     const int    nu = proj_dim[0], nv = proj_dim[1], np = proj_dim[2];
@@ -793,9 +796,17 @@ rtkhip_joseph_backproject(const int proj_dim[3], const int vol_dim[3], const dou
     RTKHIP_CHECK(hipMemsetAsync(dT.get(), 0, nvox * sizeof(float), s));
     const double infClip = 1. - superior_clip, supClip = 1. - inferior_clip;
     dim3         grid((nu + 15) / 16, (nv + 15) / 16, np);
-    hipLaunchKernelGGL(joseph_bp_tile_kernel, grid, dim3(256), 0, s, dv, nu, nv, dev_proj, dev_vol_out, dT.get(),
-                       vol_dim[0], vol_dim[1], vol_dim[2], vol_spacing[0], vol_spacing[1], vol_spacing[2], infClip,
-                       supClip);
+#define RTKHIP_JBP_LAUNCH(TP)                                                                                      \
+  hipLaunchKernelGGL(joseph_bp_tile_kernel<TP>, grid, dim3(256), 0, s, dv, nu, nv, static_cast<const TP *>(dev_proj), \
+                     dev_vol_out, dT.get(), vol_dim[0], vol_dim[1], vol_dim[2], vol_spacing[0], vol_spacing[1],       \
+                     vol_spacing[2], infClip, supClip)
+    if (proj_dtype == RTKHIP_F16)
+      RTKHIP_JBP_LAUNCH(__half);
+    else if (proj_dtype == RTKHIP_BF16)
+      RTKHIP_JBP_LAUNCH(__hip_bfloat16);
+    else
+      RTKHIP_JBP_LAUNCH(float);
+#undef RTKHIP_JBP_LAUNCH
     RTKHIP_CHECK_LAUNCH();
     dim3 tg((vol_dim[0] + 31) / 32, (vol_dim[2] + 31) / 32, vol_dim[1]);
     hipLaunchKernelGGL(add_transposed_kernel, tg, dim3(256), 0, s, dT.get(), dev_vol_out, vol_dim[0], vol_dim[1],

@@ -333,12 +333,11 @@ class HipJosephBackProjectionImageFilter(_Filter):
         vol, proj = self._inputs[0], self._inputs[1]
         _require_cuda(vol, "BP volume")
         _require_cuda(proj, "BP projections")
-        if proj.tensor.dtype != torch.float32:
-            raise ValueError("the Joseph backprojector reads FP32 projections")
+        dt = {torch.float32: _lib.F32, torch.float16: _lib.F16, torch.bfloat16: _lib.BF16}[proj.tensor.dtype]
         out = self._out_for(vol)
         vd, pd = vol.descriptor(), proj.descriptor()
         check(lib().rtkhip_joseph_back_project(_context(), self._geom(), ctypes.byref(vd), _ptr(vol.tensor),
-                                               _ptr(out), ctypes.byref(pd), _ptr(proj.tensor), _stream()))
+                                               _ptr(out), ctypes.byref(pd), _ptr(proj.tensor), dt, _stream()))
         self._output = vol.like(out)
 
 
@@ -349,7 +348,7 @@ class HipFDKBackProjectionImageFilter(HipBackProjectionImageFilter):
 
 
 class HipConjugateGradientConeBeamReconstructionFilter(_Filter):
-    """rtk::ConjugateGradientConeBeamReconstructionFilter with FP_JOSEPH / BP_VOXELBASED:
+    """rtk::ConjugateGradientConeBeamReconstructionFilter with FP_JOSEPH and BP_VOXELBASED or BP_JOSEPH:
     input 0 = X0, input 1 = projections; optional weights
     (include/rtkConjugateGradientConeBeamReconstructionFilter.hxx:195-305)."""
 
@@ -363,6 +362,14 @@ class HipConjugateGradientConeBeamReconstructionFilter(_Filter):
         self._weights = None
         self._resid = np.zeros(0)
         self._iters = 0
+        self._dtype = "f32"
+
+    def SetStackDtype(self, name):
+        """Storage of the stacks the backprojector reads: "f32", "f16" or "bf16"
+        (FP32/FP64 arithmetic in every case)."""
+        if name not in _DTYPES:
+            raise ValueError(f"unknown stack dtype {name}")
+        self._dtype = name
 
     def SetNumberOfIterations(self, n):
         self._niter = int(n)
@@ -397,7 +404,8 @@ class HipConjugateGradientConeBeamReconstructionFilter(_Filter):
         w = self._weights.tensor if self._weights is not None else None
         check(lib().rtkhip_cg_reconstruct(_context(), self._geom(), ctypes.byref(vd), _ptr(x), ctypes.byref(pd),
                                           _ptr(proj.tensor), _ptr(w), self._niter, self._tol, self._bp,
-                                          res.ctypes.data_as(_lib.PD), ctypes.byref(it), _stream()))
+                                          _DTYPES[self._dtype][0], res.ctypes.data_as(_lib.PD), ctypes.byref(it),
+                                          _stream()))
         self._iters = it.value
         self._resid = res[: it.value + 1]
         self._output = vol.like(x)

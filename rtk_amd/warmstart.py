@@ -9,6 +9,7 @@ applications/rtkconjugategradient/rtkconjugategradient.cxx:61-77), which is wher
 prediction enters.
 """
 import ctypes
+import os
 
 import numpy as np
 import torch
@@ -17,6 +18,12 @@ from . import _lib
 from ._lib import check, lib
 
 C, T = 32, 27
+
+
+def default_weights_path():
+    """The network trained by rtk_amd.warmstart_train on the sparse-view workload
+    (256^3 @1 mm, 90 views of 512^2, I0 = 1e4), shipped with the package."""
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "warmstart_c4.npz")
 
 
 def num_params():
@@ -94,6 +101,28 @@ class WarmStartCNN:
         return y[0] if squeeze else y
 
     __call__ = forward
+
+    def forward_tiled(self, x, tile=128, halo=8):
+        """Inference on a [Z, Y, X] volume in overlapping blocks: each tile^3 block is
+        run with `halo` voxels of context on every side that lies inside the volume
+        (zero padding only at the volume's own faces), and its centre is kept.  Six
+        3x3x3 layers see 6 voxels, so any halo >= 6 gives exactly the full-volume
+        result with the activations of one block resident at a time."""
+        if halo < 6:
+            raise ValueError("halo must cover the network's receptive radius (6 voxels)")
+        if x.dim() != 3:
+            raise ValueError("forward_tiled takes one [Z, Y, X] volume")
+        Z, Y, X = x.shape
+        y = torch.empty_like(x)
+        for z0 in range(0, Z, tile):
+            for y0 in range(0, Y, tile):
+                for x0 in range(0, X, tile):
+                    z1, y1, x1 = min(z0 + tile, Z), min(y0 + tile, Y), min(x0 + tile, X)
+                    za, ya, xa = max(z0 - halo, 0), max(y0 - halo, 0), max(x0 - halo, 0)
+                    zb, yb, xb = min(z1 + halo, Z), min(y1 + halo, Y), min(x1 + halo, X)
+                    out = self.forward(x[za:zb, ya:yb, xa:xb].contiguous())
+                    y[z0:z1, y0:y1, x0:x1] = out[z0 - za:z1 - za, y0 - ya:y1 - ya, x0 - xa:x1 - xa]
+        return y
 
     def loss_and_grads(self, x, target):
         """Forward + backward; returns (loss, gradient vector) without updating."""

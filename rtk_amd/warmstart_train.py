@@ -1,0 +1,194 @@
+"""Training and evaluation of the learned CG warm start on GPU-generated pairs.
+
+Sparse-view configuration (256^3 @1 mm, 90 views of 512^2 @0.8 mm, SID 1000 mm,
+SDD 1536 mm, Poisson noise I0 = 1e4): for each seed, the random-ellipsoid phantom is
+projected analytically, noise is applied, and the pair (FDK volume, volume after 100
+CG iterations from zero) is produced by this package's own HIP filters.  CG uses the
+Joseph forward projector with its matched Joseph backprojector, so the normal
+operator BP W FP is symmetric (RTK's CG with the voxel-based backprojector is not,
+and stalls at this view count).  The CNN is trained on random crops with Adam and
+evaluated on held-out seeds as CG time to a relative residual ||b - A x|| / ||b||
+of 1e-3 from three starts: zero, FDK, CNN(FDK); the start's own cost (FDK, CNN) is
+included in its time.
+
+Usage: python -m rtk_amd.warmstart_train --train 48 --eval 8 --steps 800 --out w.npz
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+import torch
+
+import rtk_amd as rtk
+from rtk_amd.warmstart import WarmStartCNN
+
+NX, NU, NVIEWS, I0 = 256, 512, 90, 1e4
+TRAIN_SEED0, EVAL_SEED0 = 1000, 5000
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def geometry(views=NVIEWS):
+    g = rtk.ThreeDCircularProjectionGeometry()
+    for i in range(views):
+        g.AddProjection(1000., 1536., i * 360. / views)
+    return g
+
+
+def volume_source(nx=NX):
+    return rtk.ConstantImageSource((nx, nx, nx), (1., 1., 1.), (-(nx - 1) / 2.,) * 3)
+
+
+def noisy_projections(g, seed, nu=NU, views=NVIEWS, i0=I0, nx=NX):
+    q = rtk.phantoms.random_ellipsoids(seed, n=20, half_fov=nx / 2.)
+    det0 = -(nu - 1) * 0.8 / 2
+    clean = rtk.project_quadrics(g, rtk.ConstantImageSource((nu, nu, views), (0.8, 0.8, 1.), (det0, det0, 0.)), q)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(int(seed))
+    counts = torch.poisson(i0 * torch.exp(-clean.tensor.double()), generator=gen)
+    return clean.like((-torch.log(torch.clamp(counts, min=1.) / i0)).float().contiguous())
+
+
+def fdk(g, proj, vol):
+    f = rtk.HipFDKConeBeamReconstructionFilter()
+    f.SetInput(0, vol)
+    f.SetInput(1, proj)
+    f.SetGeometry(g)
+    f.Update()
+    return f.GetOutput()
+
+
+def cg(g, proj, x0, niter, tol=0.):
+    f = rtk.HipConjugateGradientConeBeamReconstructionFilter()
+    f.SetInput(0, x0)
+    f.SetInput(1, proj)
+    f.SetGeometry(g)
+    f.SetBackProjectionFilter(f.BP_JOSEPH)
+    f.SetNumberOfIterations(niter)
+    f.SetResidualTolerance(tol)
+    f.Update()
+    return f.GetOutput(), f.GetResiduals(), f.GetNumberOfIterationsDone()
+
+
+def make_pair(g, seed, cg_iter=100):
+    vol = volume_source()
+    proj = noisy_projections(g, seed)
+    x_fdk = fdk(g, proj, vol)
+    x_cg, res, _ = cg(g, proj, vol, cg_iter)
+    return x_fdk.tensor, x_cg.tensor, proj, float(res[-1])
+
+
+def random_crops(pairs, crop, batch, rng):
+    xs, ts = [], []
+    for _ in range(batch):
+        x, t = pairs[rng.integers(len(pairs))][:2]
+        z, y, w = (int(rng.integers(0, NX - crop + 1)) for _ in range(3))
+        xs.append(x[z:z + crop, y:y + crop, w:w + crop])
+        ts.append(t[z:z + crop, y:y + crop, w:w + crop])
+    return torch.stack(xs).contiguous(), torch.stack(ts).contiguous()
+
+
+def train(net, pairs, steps, crop=64, batch=2, lr=1e-3, seed=0):
+    rng = np.random.default_rng(seed)
+    losses = []
+    for s in range(steps):
+        x, t = random_crops(pairs, crop, batch, rng)
+        step_lr = lr * 0.5 * (1 + np.cos(np.pi * s / steps))  # cosine decay
+        losses.append(net.train_step(x, t, lr=step_lr))
+        if (s + 1) % 100 == 0:
+            log(f"train step {s + 1}/{steps}: loss {np.mean(losses[-100:]):.4g}")
+    return losses
+
+
+def evaluate_seed(net, g, seed, tol=1e-3, max_iter=200):
+    """CG time to relative residual `tol` from zero, FDK and CNN(FDK) starts on one
+    phantom seed; the start's own cost (FDK, CNN inference) is part of its time."""
+    vol = volume_source()
+    proj = noisy_projections(g, seed)
+    torch.cuda.synchronize()
+    out = {"seed": int(seed)}
+    vols = {}
+    for name in ("zero", "fdk", "cnn"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if name == "zero":
+            x0 = vol
+        else:
+            x0 = fdk(g, proj, vol)
+            if name == "cnn":
+                x0 = x0.like(net(x0.tensor))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        x, res, it = cg(g, proj, x0, max_iter, tol)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        out[name] = {"seconds": round(t2 - t0, 4), "start_seconds": round(t1 - t0, 4), "iterations": int(it),
+                     "start_rel_residual": float(res[0]), "final_rel_residual": float(res[-1]),
+                     "reached": bool(res[-1] <= tol)}
+        vols[name] = x.tensor
+    # the end points all sit at the same residual; their spread is the solution's own slack
+    for name in ("fdk", "cnn"):
+        d = vols[name] - vols["zero"]
+        out[name]["rms_diff_vs_zero_start"] = float(torch.sqrt(torch.mean(d * d)).item())
+    return out
+
+
+def evaluate(net, g, seeds, tol=1e-3, max_iter=200):
+    """evaluate_seed over held-out seeds."""
+    rows = []
+    for seed in seeds:
+        out = evaluate_seed(net, g, seed, tol, max_iter)
+        log(f"seed {seed}: " + ", ".join(f"{k} {out[k]['iterations']} it {out[k]['seconds']:.2f} s"
+                                         for k in ("zero", "fdk", "cnn")))
+        rows.append(out)
+    return rows
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=48)
+    ap.add_argument("--eval", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=800)
+    ap.add_argument("--crop", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=2)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--cg-iter", type=int, default=100)
+    ap.add_argument("--out", default="gpurun_out/warmstart_c4.npz")
+    ap.add_argument("--report", default="gpurun_out/warmstart_c4.json")
+    args = ap.parse_args()
+    rtk.lib()
+    g = geometry()
+    t0 = time.perf_counter()
+    pairs = []
+    for i in range(args.train):
+        x, t, _, r = make_pair(g, TRAIN_SEED0 + i, args.cg_iter)
+        pairs.append((x, t))
+        if (i + 1) % 8 == 0:
+            log(f"pairs {i + 1}/{args.train} ({time.perf_counter() - t0:.0f} s), last CG residual {r:.3g}")
+    scale = float(1.0 / torch.stack([p[1].abs().mean() for p in pairs]).mean().item())
+    log(f"input scale {scale:.4g}")
+    net = WarmStartCNN(scale=scale, seed=0)
+    t1 = time.perf_counter()
+    losses = train(net, pairs, args.steps, args.crop, args.batch, args.lr)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    net.save(args.out)
+    rows = evaluate(net, g, [EVAL_SEED0 + i for i in range(args.eval)])
+    rep = {"train_pairs": args.train, "steps": args.steps, "crop": args.crop, "batch": args.batch,
+           "data_seconds": t1 - t0, "train_seconds": t2 - t1, "first_loss": float(np.mean(losses[:20])),
+           "last_loss": float(np.mean(losses[-20:])), "scale": scale, "eval": rows}
+    for name in ("zero", "fdk", "cnn"):
+        rep[f"median_{name}_seconds"] = float(np.median([r[name]["seconds"] for r in rows]))
+        rep[f"median_{name}_iterations"] = float(np.median([r[name]["iterations"] for r in rows]))
+    rep["speedup_cnn_vs_zero"] = rep["median_zero_seconds"] / rep["median_cnn_seconds"]
+    with open(args.report, "w") as f:
+        json.dump(rep, f, indent=1)
+    log(json.dumps({k: v for k, v in rep.items() if k != "eval"}))
+
+
+if __name__ == "__main__":
+    main()

@@ -64,3 +64,20 @@ def test_cnn_training_reduces_loss(hip):
     losses = [net.train_step(x, t, lr=2e-3) for _ in range(60)]
     assert np.all(np.isfinite(losses))
     assert losses[-1] < 0.5 * losses[0], (losses[0], losses[-1])
+
+
+def test_tiled_inference_equals_full_volume(hip):
+    """Overlapping tiles with an 8-voxel halo (the large-volume inference path)
+    reproduce whole-volume inference bit for bit: the 6-voxel receptive field is
+    covered and every voxel's sums run in the same order."""
+    from oracle import cnn_ref
+    from rtk_amd.warmstart import WarmStartCNN
+    p = cnn_ref.init_params(5, scale=SCALE * 20)
+    net = WarmStartCNN(params=p, scale=SCALE)
+    rng = np.random.default_rng(6)
+    x = torch.from_numpy((0.02 + 0.01 * rng.normal(size=(45, 70, 83))).astype(np.float32)).cuda()
+    full = net(x)
+    tiled = net.forward_tiled(x, tile=32, halo=8)
+    assert torch.equal(full, tiled)
+    with pytest.raises(ValueError):
+        net.forward_tiled(x, tile=32, halo=4)
