@@ -76,6 +76,50 @@ def cpu_baseline(np, rows, proj, vol, views):
                       f"views, {dt:.1f} s on {threads} threads"}
 
 
+def projector_pair_measurement(rtk, np, torch, g, q, proj, vol, reps=2):
+    """The projector pair alone at the FDK shape: Joseph forward projection of the
+    voxelised phantom (512^3) onto the 720 x 1024^2 stack, and the plain voxel-based
+    backprojection of the analytic stack into 512^3."""
+    phantom = rtk.draw_quadrics(vol, q)
+    fp = rtk.HipForwardProjectionImageFilter()
+    fp.SetInput(0, proj.like(torch.zeros_like(proj.tensor)))
+    fp.SetInput(1, phantom)
+    fp.SetGeometry(g)
+    fp.Update(count_samples=True)
+    samples = fp.GetNumberOfSamples()
+    ts = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fp.Update()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    t_fp = min(ts)
+    del fp
+    bp = rtk.HipBackProjectionImageFilter()
+    bp.SetInput(0, vol)
+    bp.SetInput(1, proj)
+    bp.SetGeometry(g)
+    bp.Update()
+    ts = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bp.Update()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    t_bp = min(ts)
+    updates = float(NX) ** 3 * NVIEWS
+    out = {"workload": "Joseph FP 512^3 -> 720 x 1024^2 and voxel BP back, same geometry as the FDK",
+           "joseph_fp": {"seconds": round(t_fp, 4), "samples": int(samples),
+                         "gsamples_per_s": round(samples / t_fp / 1e9, 2),
+                         "alg_gb_per_s": round(samples * 16 / t_fp / 1e9, 1)},
+           "voxel_bp": {"seconds": round(t_bp, 4), "gups": round(updates / t_bp / 1e9, 1),
+                        "alg_gb_per_s": round(updates * 16 / t_bp / 1e9, 1)}}
+    log(f"projector pair: {out}")
+    return out
+
+
 def cg_measurement(rtk, np, torch, max_iter, tol):
     """Plain CG on 256^3 @1 mm from 360 views of 512^2 @0.8 mm with Poisson noise
     (I0 = 1e4): 30 iterations from zero with RTK's default pair (Joseph FP + voxel
@@ -160,6 +204,7 @@ def main():
     ap.add_argument("--cpu-views", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true", help="skip the CG and warm-start measurements")
+    ap.add_argument("--no-pair", action="store_true", help="skip the Joseph FP / voxel BP measurement")
     ap.add_argument("--cg-tol", type=float, default=1e-3)
     ap.add_argument("--cg-iter", type=int, default=100)
     ap.add_argument("--warm-seeds", type=int, default=2, help="held-out seeds for the warm-start measurement")
@@ -243,6 +288,8 @@ def main():
                      "frac": round(achieved / 8000.0, 4),
                      "traffic": BP_TRAFFIC_BYTES if args.stack_dtype == "f32" else None},
     }
+    if not args.no_pair:
+        result["projector_pair"] = projector_pair_measurement(rtk, np, torch, g, q, proj, vol)
     if not args.no_cg:
         result["cg"] = cg_measurement(rtk, np, torch, args.cg_iter, args.cg_tol)
         result["cg_warm_start"] = warm_start_measurement(rtk, np, torch, _warm_seeds(args), args.cg_tol)

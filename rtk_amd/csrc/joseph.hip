@@ -20,10 +20,13 @@
 //    dominant axis); interpolation weights and sums are FP32 like the float pixels.
 //  * Rays whose dominant axis is x use a z-fastest copy of the volume (gather) or a
 //    z-fastest accumulator (scatter), so their in-slice (y, z) taps are neighbours
-//    in memory, as they are for z-dominant rays in the x-fastest layout.
+//    in memory, as they are for z-dominant rays in the x-fastest layout.  The
+//    unit-stride in-slice axis is always walked as the first one, so each pair of
+//    taps along it is one 8-byte load.
 //  * The backprojector is the exact transpose of the forward kernel's sampling (same
-//    set-up code, same taps and weights) with hardware FP32 atomics, which makes
-//    BP_Joseph W FP_Joseph symmetric for conjugate gradient.
+//    set-up code, same taps and weights), summed per ray tile in LDS and added with
+//    FP32 atomics, which makes BP_Joseph W FP_Joseph symmetric for conjugate
+//    gradient.
 #include "common.h"
 
 #include <hip/amd_detail/amd_hip_unsafe_atomics.h>
@@ -101,11 +104,23 @@ __global__ __launch_bounds__(256) void add_transposed_kernel(const float * __res
   }
 }
 
+// In-slice axes of a ray whose dominant axis is mainDir, the unit-stride one first:
+// x for y- and z-dominant rays (x-fastest volume), z for x-dominant rays (z-fastest
+// copy).  Tap pairs along the first axis are then adjacent floats, which the
+// compiler loads as one 8-byte access.  Each axis is clamped and interpolated on
+// its own, so the order only changes the FP32 rounding of the 4-tap sum.
+__host__ __device__ __forceinline__ void
+InSliceAxes(int mainDir, int & dA, int & dB)
+{
+  dA = mainDir == 0 ? 2 : 0;
+  dB = mainDir == 1 ? 2 : 1;
+}
+
 // Everything a ray needs to walk its samples.
 struct Ray
 {
   int    ns, fs, nmid;   // first/last slice, number of middle steps
-  int    ox, oy, oz;     // in-slice and slice offsets (signed slice offset)
+  int    oy, oz;         // second in-slice offset and signed slice offset (the first is 1)
   int    start;          // offset of slice ns (ns times the unsigned slice offset)
   int    nxm2, nym2;     // in-slice index limits (N - 2)
   bool   transposed;     // dominant axis x: z-fastest buffer
@@ -193,20 +208,14 @@ SetupRay(const JosephView & V, int iu, int jv, int p, int nx, int ny, int nz, do
   R.ns = int(round(npt[mainDir])); // itk::Math::rnd, half away from zero
   R.fs = int(round(fpt[mainDir]));
   R.nmid = abs(R.fs - R.ns) - 1;
-  int dInf = (mainDir + 1) % 3, dSup = (mainDir + 2) % 3;
-  if (dInf > dSup)
-  {
-    const int t = dInf;
-    dInf = dSup;
-    dSup = t;
-  }
+  int dInf, dSup;
+  InSliceAxes(mainDir, dInf, dSup);
   R.transposed = useT;
   R.axis = mainDir;
   R.maxx = bmax[dInf];
   R.maxy = bmax[dSup];
   R.nxm2 = n3[dInf] - 2;
   R.nym2 = n3[dSup] - 2;
-  R.ox = off3[dInf];
   R.oy = off3[dSup];
   R.oz = off3[mainDir];
   R.start = R.ns * R.oz; // before the sign of the step is applied
@@ -265,7 +274,7 @@ BorderTaps(const Ray & R, double x, double y, int t[4], float w[4])
   BorderTaps2D(R, x, y, tx, ty, w);
 #pragma unroll
   for (int q = 0; q < 4; q++)
-    t[q] = tx[q] * R.ox + ty[q] * R.oy;
+    t[q] = tx[q] + ty[q] * R.oy;
 }
 
 // Middle steps: the in-slice position as integer index + FP64 fraction.  Middle
@@ -286,7 +295,7 @@ struct Walker
   {
     int cix, ciy;
     Stage2D(R, cix, ciy, lx, ly);
-    return int(__umul24(unsigned(cix), unsigned(R.ox)) + __umul24(unsigned(ciy), unsigned(R.oy)));
+    return cix + int(__umul24(unsigned(ciy), unsigned(R.oy)));
   }
   // Lower-left tap (cix, ciy) of the current sample, then advance one slice.
   __device__ __forceinline__ void Stage2D(const Ray & R, int & cix, int & ciy, float & lx, float & ly)
@@ -349,8 +358,8 @@ __global__ __launch_bounds__(256) void joseph_kernel(const JosephView * __restri
           float       lx, ly;
           const int   idx = wk.Stage(R, lx, ly);
           const float lxc = 1.f - lx, lyc = 1.f - ly;
-          sum += (lxc * lyc) * b[idx] + (lx * lyc) * b[idx + R.ox] + (lxc * ly) * b[idx + R.oy] +
-                 (lx * ly) * b[idx + R.ox + R.oy];
+          sum += (lxc * lyc) * b[idx] + (lx * lyc) * b[idx + 1] + (lxc * ly) * b[idx + R.oy] +
+                 (lx * ly) * b[idx + 1 + R.oy];
           b += R.oz;
         }
         sum += border(R.wE, b, wk.ix + wk.fx, wk.iy + wk.fy);
@@ -404,9 +413,9 @@ SplatRay(const Ray & R, float value, float * acc, float * accT)
     const int   idx = wk.Stage(R, lx, ly);
     const float lxc = 1.f - lx, lyc = 1.f - ly;
     unsafeAtomicAdd(b + idx, s * (lxc * lyc));
-    unsafeAtomicAdd(b + idx + R.ox, s * (lx * lyc));
+    unsafeAtomicAdd(b + idx + 1, s * (lx * lyc));
     unsafeAtomicAdd(b + idx + R.oy, s * (lxc * ly));
-    unsafeAtomicAdd(b + idx + R.ox + R.oy, s * (lx * ly));
+    unsafeAtomicAdd(b + idx + 1 + R.oy, s * (lx * ly));
     b += R.oz;
   }
   border(R.wE, b, wk.ix + wk.fx, wk.iy + wk.fy);
@@ -490,13 +499,8 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
     // non-negative FP32 bit patterns order like the values; NaN and Inf sort above
     atomicMax(&maxBits, __float_as_int(fabsf(float(double(value) * R.stepNorm))));
   }
-  int dInf = (axis + 1) % 3, dSup = (axis + 2) % 3;
-  if (dInf > dSup)
-  {
-    const int t = dInf;
-    dInf = dSup;
-    dSup = t;
-  }
+  int dInf, dSup;
+  InSliceAxes(axis, dInf, dSup);
   if (tid < 4)
   {
     // corner pixels of the part of the tile inside the detector
@@ -565,7 +569,7 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
   float *      buf = useT ? accT : acc;
   const int    n3[3] = { nx, ny, nz };
   const int    off3[3] = { useT ? ny * nz : 1, useT ? nz : nx, useT ? 1 : nx * ny };
-  const int    ox = off3[dInf], oy = off3[dSup], oz = off3[axis];
+  const int    oy = off3[dSup], oz = off3[axis]; // off3[dInf] == 1
   const int    NX = n3[dInf], NY = n3[dSup];
   const int    r0 = part ? (R.ns - kfirst) * dirSign : INT_MAX; // this ray's first and last tile step
   const int    r1 = part ? (R.fs - kfirst) * dirSign : INT_MIN;
@@ -639,27 +643,26 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
           if (unsigned(a) < unsigned(kTileWin) && unsigned(b) < unsigned(kTileWin))
             atomicAdd(&w[b * kTileStride + a], (unsigned long long)__double2ll_rn(double(v) * toFixed));
           else
-            unsafeAtomicAdd(g + tx[q] * ox + ty[q] * oy, v);
+            unsafeAtomicAdd(g + tx[q] + ty[q] * oy, v);
         }
       }
     }
     if (s > 0)
     {
-      // flush slice s - 1; lanes run along the unit-stride in-slice axis
+      // flush slice s - 1; lanes run along the unit-stride in-slice axis (window x)
       const int k1 = k - dirSign;
       unsigned long long * w = win[(s - 1) & 1];
       float *              g = buf + size_t(k1) * oz;
       for (int c = tid; c < kTileWin * kTileWin; c += 256)
       {
-        const int                a = c & (kTileWin - 1), b = c / kTileWin;
-        const int                i = (ox == 1) ? a : b, j = (ox == 1) ? b : a;
+        const int                i = c & (kTileWin - 1), j = c / kTileWin;
         const unsigned long long q = w[j * kTileStride + i];
         if (q != 0ull)
         {
           w[j * kTileStride + i] = 0ull;
           const int X = pwx + i, Y = pwy + j;
           if (X >= 0 && X < NX && Y >= 0 && Y < NY)
-            unsafeAtomicAdd(g + X * ox + Y * oy, float(double((long long)q) * fromFixed));
+            unsafeAtomicAdd(g + X + Y * oy, float(double((long long)q) * fromFixed));
         }
       }
     }

@@ -50,6 +50,7 @@ class ThreeDCircularProjectionGeometry:
             raise _lib.RtkHipError(_lib.RTKHIP_ERR_OUT_OF_MEMORY, "geometry allocation failed")
         self._rows = []
         self._radius = 0.
+        self.collimation = []  # per view: UInf, USup, VInf, VSup (field-of-view filters only)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -63,6 +64,14 @@ class ThreeDCircularProjectionGeometry:
                                   inPlaneAngle, sourceOffsetX, sourceOffsetY)]
         check(lib().rtkhip_geometry_add_projection(self._h, *row))
         self._rows.append(row)
+        self.collimation.append([np.finfo(np.float64).max] * 4)
+
+    def SetCollimationOfLastProjection(self, uinf, usup, vinf, vsup):
+        """Stored and written with the geometry; the projectors of this path do not use it
+        (the reference reads it in its field-of-view and collimation-mask filters)."""
+        if not self.collimation:
+            raise ValueError("no projection to set the collimation of")
+        self.collimation[-1] = [float(uinf), float(usup), float(vinf), float(vsup)]
 
     def SetRadiusCylindricalDetector(self, r):
         check(lib().rtkhip_geometry_set_radius_cylindrical_detector(self._h, float(r)))

@@ -12,6 +12,7 @@ of 1e-3 from three starts: zero, FDK, CNN(FDK); the start's own cost (FDK, CNN) 
 included in its time.
 
 Usage: python -m rtk_amd.warmstart_train --train 48 --eval 8 --steps 800 --out w.npz
+       python -m rtk_amd.warmstart_train --scale-eval 1   (512^3 / 180 views, tiled CNN)
 """
 import argparse
 import json
@@ -148,6 +149,50 @@ def evaluate(net, g, seeds, tol=1e-3, max_iter=200):
     return rows
 
 
+def evaluate_at_scale(net, seed, tol=1e-3, max_iter=300, tile=128, halo=8):
+    """The learned start at scale: 512^3 @0.5 mm from 180 views of 1024^2 @0.4 mm
+    (SID 1000 mm, SDD 1536 mm), 30 random ellipsoids, Poisson noise I0 = 5e3.  The
+    network runs as overlapping tile^3 blocks with `halo` voxels of context, then CG
+    (Joseph pair, FP64 scalars) runs to relative residual `tol` from zero, FDK and
+    CNN(FDK).  Returns the same fields as evaluate_seed."""
+    views, nu, nx = 180, 1024, 512
+    g = geometry(views)
+    vol = rtk.ConstantImageSource((nx, nx, nx), (0.5, 0.5, 0.5), (-(nx - 1) * 0.5 / 2,) * 3)
+    q = rtk.phantoms.random_ellipsoids(seed, n=30, half_fov=nx * 0.5 / 2)
+    det0 = -(nu - 1) * 0.4 / 2
+    clean = rtk.project_quadrics(g, rtk.ConstantImageSource((nu, nu, views), (0.4, 0.4, 1.), (det0, det0, 0.)), q)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(int(seed))
+    counts = torch.poisson(5e3 * torch.exp(-clean.tensor.double()), generator=gen)
+    proj = clean.like((-torch.log(torch.clamp(counts, min=1.) / 5e3)).float().contiguous())
+    del clean, counts
+    out = {"seed": int(seed)}
+    vols = {}
+    for name in ("zero", "fdk", "cnn"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if name == "zero":
+            x0 = vol
+        else:
+            x0 = fdk(g, proj, vol)
+            if name == "cnn":
+                x0 = x0.like(net.forward_tiled(x0.tensor, tile=tile, halo=halo))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        x, res, it = cg(g, proj, x0, max_iter, tol)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        out[name] = {"seconds": round(t2 - t0, 3), "start_seconds": round(t1 - t0, 4), "iterations": int(it),
+                     "start_rel_residual": float(res[0]), "final_rel_residual": float(res[-1]),
+                     "reached": bool(res[-1] <= tol)}
+        vols[name] = x.tensor
+        log(f"scale seed {seed} {name}: {out[name]}")
+    for name in ("fdk", "cnn"):
+        d = vols[name] - vols["zero"]
+        out[name]["rms_diff_vs_zero_start"] = float(torch.sqrt(torch.mean(d * d)).item())
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=48)
@@ -159,8 +204,17 @@ def main():
     ap.add_argument("--cg-iter", type=int, default=100)
     ap.add_argument("--out", default="gpurun_out/warmstart_c4.npz")
     ap.add_argument("--report", default="gpurun_out/warmstart_c4.json")
+    ap.add_argument("--scale-eval", type=int, default=0,
+                    help="only evaluate the shipped network at 512^3 / 180 views on this many held-out seeds")
     args = ap.parse_args()
     rtk.lib()
+    if args.scale_eval:
+        from rtk_amd.warmstart import default_weights_path
+        net = WarmStartCNN.load(default_weights_path())
+        rows = [evaluate_at_scale(net, EVAL_SEED0 + 100 + i) for i in range(args.scale_eval)]
+        print(json.dumps({"workload": "CG 512^3 @0.5 mm <- 180 x 1024^2 @0.4 mm, 30 ellipsoids, I0=5e3",
+                          "eval": rows}), flush=True)
+        return
     g = geometry()
     t0 = time.perf_counter()
     pairs = []
