@@ -438,9 +438,10 @@ SplatRay(const Ray & R, float value, float * acc, float * accT)
 // of the tile: a central projection maps the pixel rectangle onto a convex
 // quadrilateral of the slice plane, so the corners bound every ray's position; its
 // width is a convex function of k, so checking the first and last slice bounds all.
-// Taps that still fall outside (rounding), rays of a minority axis or direction, and
-// tiles whose footprint exceeds the window or that hold a non-finite value take the
-// per-ray path.
+// Slices whose footprint fits 16 x 16 cells use (and flush) only that corner of the
+// window, one cell per lane.  Taps that still fall outside (rounding), rays of a
+// minority axis or direction, and tiles whose footprint exceeds the window or that
+// hold a non-finite value take the per-ray path.
 constexpr int kTileWin = 32;    // LDS window edge, in voxels
 constexpr int kTileStride = 33; // LDS row stride, in cells
 
@@ -455,7 +456,7 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
   __shared__ int    vote[6];
   __shared__ int    maxBits; // largest |value * stepNorm| of the participating rays, as FP32 bits
   __shared__ int    krange[2]; // min and max slice over the tile's participating rays
-  __shared__ double line[4][4]; // corner c: in-slice x = line[c][0] + k line[c][1], y likewise
+  __shared__ float  line[4][4]; // corner c: in-slice x = line[c][0] + k line[c][1], y likewise
   __shared__ int    cornerOk;
 
   const int tid = threadIdx.x;
@@ -518,22 +519,24 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
     else
     {
       const double bx = dir[dInf] / dm, by = dir[dSup] / dm;
-      line[tid][0] = pos[dInf] - pos[axis] * bx;
-      line[tid][1] = bx;
-      line[tid][2] = pos[dSup] - pos[axis] * by;
-      line[tid][3] = by;
+      // FP32 is enough to place a window with a one-voxel margin; taps that still
+      // fall outside take the global path
+      line[tid][0] = float(pos[dInf] - pos[axis] * bx);
+      line[tid][1] = float(bx);
+      line[tid][2] = float(pos[dSup] - pos[axis] * by);
+      line[tid][3] = float(by);
     }
   }
   __syncthreads();
   const int   kmin = krange[0], kmax = krange[1];
   const float maxc = __int_as_float(maxBits);
   bool        tiled = cornerOk && kmin <= kmax && maxc <= FLT_MAX;
-  auto      window = [&](int k, int & wx0, int & wy0, double & spanx, double & spany) {
-    double lox = DBL_MAX, hix = -DBL_MAX, loy = DBL_MAX, hiy = -DBL_MAX;
+  auto      window = [&](int k, int & wx0, int & wy0, float & spanx, float & spany) {
+    float lox = FLT_MAX, hix = -FLT_MAX, loy = FLT_MAX, hiy = -FLT_MAX;
 #pragma unroll
     for (int c = 0; c < 4; c++)
     {
-      const double x = line[c][0] + k * line[c][1], y = line[c][2] + k * line[c][3];
+      const float x = line[c][0] + k * line[c][1], y = line[c][2] + k * line[c][3];
       lox = fmin(lox, x);
       hix = fmax(hix, x);
       loy = fmin(loy, y);
@@ -546,12 +549,12 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
   };
   if (tiled)
   {
-    int    wx, wy;
-    double sx0, sy0, sx1, sy1;
+    int   wx, wy;
+    float sx0, sy0, sx1, sy1;
     window(kmin, wx, wy, sx0, sy0);
     window(kmax, wx, wy, sx1, sy1);
     // taps reach floor(hi) + 1, the window starts at floor(lo) - 1: span + 4 cells
-    tiled = fmax(sx0, sx1) + 5. <= kTileWin && fmax(sy0, sy1) + 5. <= kTileWin;
+    tiled = fmaxf(sx0, sx1) + 5.f <= kTileWin && fmaxf(sy0, sy1) + 5.f <= kTileWin;
   }
   if (!tiled)
   {
@@ -580,14 +583,17 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
   const double toFixed = ldexp(1., 50 - emax), fromFixed = ldexp(1., emax - 50);
   Walker       wk;
   int          pwx = 0, pwy = 0; // window origin of the previous slice
+  int          plim = kTileWin;  // cells per window edge flushed for the previous slice
   for (int s = 0; s <= nsl; ++s)
   {
     const int k = kfirst + s * dirSign;
     int       wx = 0, wy = 0;
+    int       lim = kTileWin; // window edge used for this slice: 16 when the footprint fits
     if (s < nsl)
     {
-      double sx, sy;
+      float sx, sy;
       window(k, wx, wy, sx, sy);
+      lim = (sx + 5.f <= 16.f && sy + 5.f <= 16.f) ? 16 : kTileWin;
       if (s >= r0 && s <= r1)
       {
         unsigned long long * w = win[s & 1];
@@ -640,7 +646,7 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
         {
           const int   a = tx[q] - wx, b = ty[q] - wy;
           const float v = sc * tw[q];
-          if (unsigned(a) < unsigned(kTileWin) && unsigned(b) < unsigned(kTileWin))
+          if (unsigned(a) < unsigned(lim) && unsigned(b) < unsigned(lim))
             atomicAdd(&w[b * kTileStride + a], (unsigned long long)__double2ll_rn(double(v) * toFixed));
           else
             unsafeAtomicAdd(g + tx[q] + ty[q] * oy, v);
@@ -653,9 +659,11 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
       const int k1 = k - dirSign;
       unsigned long long * w = win[(s - 1) & 1];
       float *              g = buf + size_t(k1) * oz;
-      for (int c = tid; c < kTileWin * kTileWin; c += 256)
+      // a 16 x 16 window is one cell per lane; a 32 x 32 one four
+      const int shift = plim == 16 ? 4 : 5;
+      for (int c = tid; c < plim * plim; c += 256)
       {
-        const int                i = c & (kTileWin - 1), j = c / kTileWin;
+        const int                i = c & (plim - 1), j = c >> shift;
         const unsigned long long q = w[j * kTileStride + i];
         if (q != 0ull)
         {
@@ -668,6 +676,7 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
     }
     pwx = wx;
     pwy = wy;
+    plim = lim;
     __syncthreads();
   }
 }
