@@ -11,9 +11,11 @@
 //    Each thread owns a column of NJ voxels along y at fixed (x, z) and keeps
 //    them in registers for the whole launch; per view it computes u, w and the
 //    row step once (one reciprocal) and then only steps v along the column.
-//  * A wavefront covers 64 consecutive x at one z: its bilinear taps fall in
-//    a narrow band of detector columns, so gathers coalesce into few L1 lines;
-//    the 4 wavefronts of a workgroup cover neighbouring z and share those lines.
+//  * A workgroup is 16 (x) x 16 (z) such columns.  Per view it stages the detector
+//    rectangle its voxel block projects onto (bounded by the block's corners) in
+//    LDS with 16-byte row loads, double-buffered, and reads its bilinear taps from
+//    there: the gathers no longer go through the texture-address path, which
+//    bounds a global-memory gather kernel (4 lane-loads per update).
 //  * Many views per launch: the volume is read and written once per launch,
 //    the per-view matrices are wave-uniform (scalar loads).
 //  * Everything else (X-path views, ITK generic interpolation, non-zero
@@ -23,7 +25,9 @@
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
+#include <cfloat>
 #include <cmath>
+#include <type_traits>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -40,73 +44,281 @@ struct alignas(16) ViewCoef
   float c[12];
 };
 
-constexpr int kBlockX = 64; // one wavefront along x
-constexpr int kBlockZ = 4;  // 4 wavefronts along z
+// Column kernel, for views whose u and w do not depend on y (c[2] == c[10] == 0)
+// and that use the OPTIMIZED border rule (RTK's OptimizedBackprojectionY case).
+constexpr int kTileU = 64, kTileV = 64, kTileStride = 65; // staged detector footprint, pixels
 
-// Fast column kernel: requires c[2] == c[10] == 0 (u and w independent of y)
-// and the OPTIMIZED border rule for every view of the launch.
-template <typename TP, bool FDK, int NJ>
-__global__ __launch_bounds__(kBlockX * kBlockZ) void bp_column_kernel(const ViewCoef * __restrict__ views,
-                                                                        int nviews, const TP * __restrict__ proj,
-                                                                        int nu, int nv, const float * vol_in,
-                                                                        float * vol_out, int nx, int ny, int nz,
-                                                                        float ci, float cj, float ck)
+// Detector rectangle [u0, u0 + W) x [v0, v0 + H) holding every tap of the voxel block
+// [xlo, xhi] x [ylo, yhi] x [zlo, zhi] (centred coordinates) for view c.  u, v are
+// linear-fractional in the voxel position; where the denominator w keeps one sign
+// over the block (positive for the FDK's normalised matrices, negative for the plain
+// backprojector's) the block's corners bound them.  One pixel of margin covers
+// rounding.  W = 0: no tap can pass the border rule (skip the view); W < 0: too
+// large, or w changes sign (read global memory for this view).
+__device__ __forceinline__ void
+Footprint(const ViewCoef & c, float xlo, float xhi, float ylo, float yhi, float zlo, float zhi, int nu, int nv,
+          int & u0, int & v0, int & W, int & H)
 {
-  const int i = blockIdx.x * kBlockX + threadIdx.x;
-  const int k = blockIdx.y * kBlockZ + threadIdx.y;
-  const int j0 = blockIdx.z * NJ;
-  if (i >= nx || k >= nz)
+  float umin = FLT_MAX, umax = -FLT_MAX, vmin = FLT_MAX, vmax = -FLT_MAX;
+  bool  pos = true, neg = true;
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+  {
+    const float x = (q & 1) ? xhi : xlo, z = (q & 2) ? zhi : zlo;
+    const float w = c.c[8] + c.c[9] * x + c.c[11] * z;
+    pos = pos && (w > 0.f);
+    neg = neg && (w < 0.f);
+    const float rw = 1.f / w;
+    const float u = (c.c[0] + c.c[1] * x + c.c[3] * z) * rw;
+    const float vl = (c.c[4] + c.c[5] * x + c.c[7] * z + c.c[6] * ylo) * rw;
+    const float vh = (c.c[4] + c.c[5] * x + c.c[7] * z + c.c[6] * yhi) * rw;
+    umin = fminf(umin, u);
+    umax = fmaxf(umax, u);
+    vmin = fminf(vmin, fminf(vl, vh));
+    vmax = fmaxf(vmax, fmaxf(vl, vh));
+  }
+  u0 = v0 = 0;
+  H = 0;
+  if (!(pos || neg) || !(umin <= umax) || !(vmin <= vmax) || !(umax - umin < 1e6f) || !(vmax - vmin < 1e6f))
+  {
+    W = -1;
     return;
-  const long   sliceStride = long(nx) * ny;
-  const long   base = long(k) * sliceStride + long(j0) * nx + i;
-  const int    jn = min(NJ, ny - j0);
-  float        acc[NJ];
+  }
+  const float fuLo = floorf(fmaxf(umin, -2.f)), fuHi = floorf(fminf(umax, float(nu)));
+  const float fvLo = floorf(fmaxf(vmin, -2.f)), fvHi = floorf(fminf(vmax, float(nv)));
+  if (fuHi < 0.f || fuLo > float(nu - 2) || fvHi < 0.f || fvLo > float(nv - 2))
+  {
+    W = 0;
+    return;
+  }
+  u0 = max(int(fuLo) - 1, 0) & ~3;
+  v0 = max(int(fvLo) - 1, 0);
+  const int u1 = min(int(fuHi) + 2, nu - 1), v1 = min(int(fvHi) + 2, nv - 1);
+  W = u1 - u0 + 1;
+  H = v1 - v0 + 1;
+  if (W > kTileU || H > kTileV)
+    W = -1;
+}
+
+// Each thread accumulates a column of NJ voxels along y; a workgroup is 16 (x) x 16
+// (z) columns, a wavefront an 8 x 8 (x, z) patch.  Each view's detector footprint of
+// the workgroup's voxel block is staged in LDS: the footprint of view p + 1 is loaded
+// into registers before view p is backprojected from LDS and written to the other
+// buffer after it, so there is one barrier per view and the loads overlap the
+// arithmetic.  Taps outside the staged rectangle (rounding) and views whose
+// footprint does not fit read global memory; results are identical either way.
+template <typename TP, bool FDK, int NJ>
+__global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict__ views, int nviews,
+                                                     const TP * __restrict__ proj, int nu, int nv,
+                                                     const float * vol_in, float * vol_out, int nx, int ny, int nz,
+                                                     float ci, float cj, float ck)
+{
+  __shared__ float stage[2][kTileV * kTileStride];
+  __shared__ int   rects[2][4]; // footprints of the next two views (u0, v0, W, H), by thread 0
+  const int        t = threadIdx.x;
+  const int        bx0 = blockIdx.x * 16, bz0 = blockIdx.y * 16, j0 = blockIdx.z * NJ;
+  const int        i = bx0 + (t & 7) + 8 * ((t >> 6) & 1);
+  const int        k = bz0 + ((t >> 3) & 7) + 8 * (t >> 7);
+  const bool       live = i < nx && k < nz;
+  const long       sliceStride = long(nx) * ny;
+  const long       base = long(k) * sliceStride + long(j0) * nx + i;
+  const int        jn = min(NJ, ny - j0);
+  float            acc[NJ];
 #pragma unroll
   for (int jj = 0; jj < NJ; jj++)
-    acc[jj] = (jj < jn && vol_in) ? vol_in[base + long(jj) * nx] : 0.f;
+    acc[jj] = (live && jj < jn && vol_in) ? vol_in[base + long(jj) * nx] : 0.f;
 
-  const float  x = float(i) - ci;
-  const float  z = float(k) - ck;
-  const float  y0 = float(j0) - cj;
-  const long   projStride = long(nu) * nv;
-  const float  numax = float(nu - 1);
-  for (int p = 0; p < nviews; p++)
-  {
-    const ViewCoef & c = views[p];
-    const float      w = c.c[8] + c.c[9] * x + c.c[11] * z;
-    const float      rw = 1.f / w;
-    const float      u = (c.c[0] + c.c[1] * x + c.c[3] * z) * rw;
-    const float      uf = floorf(u);
-    if (!(uf >= 0.f && uf < numax))
-      continue;
-    const int   ui = int(uf);
-    const float fu = u - uf;
-    const float vb = (c.c[4] + c.c[5] * x + c.c[7] * z + c.c[6] * y0) * rw;
-    const float dv = c.c[6] * rw;
-    const float wgt = FDK ? rw * rw : 1.f;
-    const TP *  pp = proj + long(p) * projStride + ui;
+  const float x = float(i) - ci, z = float(k) - ck, y0 = float(j0) - cj;
+  const float xlo = float(bx0) - ci, xhi = float(min(bx0 + 15, nx - 1)) - ci;
+  const float zlo = float(bz0) - ck, zhi = float(min(bz0 + 15, nz - 1)) - ck;
+  const float ylo = y0, yhi = float(j0 + jn - 1) - cj;
+  const long  projStride = long(nu) * nv;
+  const float numax = float(nu - 1);
+  // staging: lane = 4 consecutive pixels (one 16-byte load for FP32 when rows are
+  // 4-aligned), 16 lanes per 64-pixel row, 16 rows per pass, 4 passes
+  const int   sc = 4 * (t & 15), sr = t >> 4;
+  const bool  vec = (nu & 3) == 0 && (reinterpret_cast<size_t>(proj) & 15) == 0;
+
+  float sreg[16];
+  int   cu0, cv0, cW, cH; // footprint of the view being backprojected
+  auto  load = [&](int p, int u0, int v0, int W, int H) {
+    const long o0 = long(p) * projStride + long(v0 + sr) * nu + (u0 + sc);
 #pragma unroll
-    for (int jj = 0; jj < NJ; jj++)
+    for (int q = 0; q < 4; q++)
     {
-      const float vv = vb + float(jj) * dv;
-      const float vf = floorf(vv);
-      const int   vi = int(vf);
-      if (vf >= 0.f && vi < nv - 1)
+      const int  r = sr + 16 * q;
+      const long o = o0 + long(16 * q) * nu;
+      if (vec && r < H && sc < W && u0 + sc + 3 <= nu - 1)
       {
-        const float fv = vv - vf;
-        const long  o = long(vi) * nu;
-        const float a = LoadPix(pp, o), b = LoadPix(pp, o + nu);
-        const float cc = LoadPix(pp, o + 1), d = LoadPix(pp, o + nu + 1);
-        const float t0 = (1.f - fv) * a + fv * b;
-        const float t1 = (1.f - fv) * cc + fv * d;
-        acc[jj] += wgt * ((1.f - fu) * t0 + fu * t1);
+        if constexpr (std::is_same<TP, float>::value)
+        {
+          typedef float f4 __attribute__((ext_vector_type(4)));
+          const f4 f = *reinterpret_cast<const f4 *>(proj + o);
+          sreg[4 * q] = f.x;
+          sreg[4 * q + 1] = f.y;
+          sreg[4 * q + 2] = f.z;
+          sreg[4 * q + 3] = f.w;
+        }
+        else
+        {
+#pragma unroll
+          for (int e = 0; e < 4; e++)
+            sreg[4 * q + e] = LoadPix(proj, o + e);
+        }
+      }
+      else
+      {
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+          sreg[4 * q + e] = (r < H && sc + e < W) ? LoadPix(proj, o + e) : 0.f;
       }
     }
-  }
+  };
+  // whole 4-pixel chunks are stored: cells at or beyond W hold pixels (or zeros) that
+  // no tap reads, and sc + 3 <= 63 keeps them inside the tile
+  auto store = [&](int b, int W, int H) {
 #pragma unroll
-  for (int jj = 0; jj < NJ; jj++)
-    if (jj < jn)
-      vol_out[base + long(jj) * nx] = acc[jj];
+    for (int q = 0; q < 4; q++)
+    {
+      const int r = sr + 16 * q;
+      if (r < H && sc < W)
+      {
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+          stage[b][r * kTileStride + sc + e] = sreg[4 * q + e];
+      }
+    }
+  };
+  // thread 0 places the footprint of view q two views ahead, in rects[q & 1]; the
+  // slot it overwrites was read before the barrier that precedes the write
+  auto publish = [&](int q) {
+    int r[4];
+    Footprint(views[q], xlo, xhi, ylo, yhi, zlo, zhi, nu, nv, r[0], r[1], r[2], r[3]);
+    for (int e = 0; e < 4; e++)
+      rects[q & 1][e] = r[e];
+  };
+  if (t == 0)
+  {
+    publish(0);
+    if (nviews > 1)
+      publish(1);
+  }
+  __syncthreads();
+  cu0 = rects[0][0];
+  cv0 = rects[0][1];
+  cW = rects[0][2];
+  cH = rects[0][3];
+  if (cW > 0)
+  {
+    load(0, cu0, cv0, cW, cH);
+    store(0, cW, cH);
+  }
+  __syncthreads();
+  for (int p = 0; p < nviews; p++)
+  {
+    const int b = p & 1;
+    int       nu0 = 0, nv0 = 0, nW = 0, nH = 0;
+    if (p + 1 < nviews)
+    {
+      nu0 = rects[(p + 1) & 1][0];
+      nv0 = rects[(p + 1) & 1][1];
+      nW = rects[(p + 1) & 1][2];
+      nH = rects[(p + 1) & 1][3];
+      if (nW > 0)
+        load(p + 1, nu0, nv0, nW, nH);
+    }
+    if (t == 0 && p + 2 < nviews)
+      publish(p + 2);
+    if (live && cW != 0)
+    {
+      const ViewCoef & c = views[p];
+      const float      w = c.c[8] + c.c[9] * x + c.c[11] * z;
+      const float      rw = 1.f / w;
+      const float      u = (c.c[0] + c.c[1] * x + c.c[3] * z) * rw;
+      const float      uf = floorf(u);
+      if (uf >= 0.f && uf < numax)
+      {
+        const int    ui = int(uf);
+        const float  fu = u - uf;
+        const float  vb = (c.c[4] + c.c[5] * x + c.c[7] * z + c.c[6] * y0) * rw;
+        const float  dv = c.c[6] * rw;
+        const float  wgt = FDK ? rw * rw : 1.f;
+        const int    lc = ui - cu0;
+        const float * S = stage[b];
+        const TP *   pp = proj + long(p) * projStride + ui;
+        // v is monotonic along the column (one FMA per voxel), so its two ends bound
+        // every tap: when both pass the border rule and lie in the staged rectangle,
+        // the whole column runs without per-voxel tests
+        const float  vlast = vb + float(NJ - 1) * dv;
+        const float  flo = floorf(fminf(vb, vlast)), fhi = floorf(fmaxf(vb, vlast));
+        const bool   fast = jn == NJ && cW > 0 && unsigned(lc) < unsigned(cW - 1) && flo >= 0.f &&
+                          fhi < float(nv - 1) && int(flo) >= cv0 && int(fhi) - cv0 < cH - 1;
+        if (fast)
+        {
+          const int    off = lc - cv0 * kTileStride;
+          const float  w0 = wgt * (1.f - fu), w1 = wgt * fu;
+#pragma unroll
+          for (int jj = 0; jj < NJ; jj++)
+          {
+            const float   vv = vb + float(jj) * dv;
+            const float   vf = floorf(vv);
+            const float   fv = vv - vf;
+            const float * s0 = S + (int(vf) * kTileStride + off);
+            const float   t0 = s0[0] + fv * (s0[kTileStride] - s0[0]);
+            const float   t1 = s0[1] + fv * (s0[kTileStride + 1] - s0[1]);
+            acc[jj] += w0 * t0 + w1 * t1;
+          }
+        }
+        else
+#pragma unroll
+        for (int jj = 0; jj < NJ; jj++)
+        {
+          const float vv = vb + float(jj) * dv;
+          const float vf = floorf(vv);
+          const int   vi = int(vf);
+          if (vf >= 0.f && vi < nv - 1)
+          {
+            const float fv = vv - vf;
+            const int   lr = vi - cv0;
+            float       a, bb, cc, d;
+            if (cW > 0 && unsigned(lc) < unsigned(cW - 1) && unsigned(lr) < unsigned(cH - 1))
+            {
+              const float * s0 = S + lr * kTileStride + lc;
+              a = s0[0];
+              cc = s0[1];
+              bb = s0[kTileStride];
+              d = s0[kTileStride + 1];
+            }
+            else
+            {
+              const long o = long(vi) * nu;
+              a = LoadPix(pp, o);
+              bb = LoadPix(pp, o + nu);
+              cc = LoadPix(pp, o + 1);
+              d = LoadPix(pp, o + nu + 1);
+            }
+            const float t0 = (1.f - fv) * a + fv * bb;
+            const float t1 = (1.f - fv) * cc + fv * d;
+            acc[jj] += wgt * ((1.f - fu) * t0 + fu * t1);
+          }
+        }
+      }
+    }
+    if (nW > 0)
+      store(b ^ 1, nW, nH);
+    __syncthreads();
+    cu0 = nu0;
+    cv0 = nv0;
+    cW = nW;
+    cH = nH;
+  }
+  if (live)
+  {
+#pragma unroll
+    for (int jj = 0; jj < NJ; jj++)
+      if (jj < jn)
+        vol_out[base + long(jj) * nx] = acc[jj];
+  }
 }
 
 // itk::LinearInterpolateImageFunction (2-D) on the projection, in the
@@ -206,9 +418,8 @@ Launch(bool column, const ViewCoef * dcoef, const int * dborder, int nviews, con
   const TP * p = static_cast<const TP *>(proj);
   if (column)
   {
-    dim3 block(kBlockX, kBlockZ, 1);
-    dim3 grid((nx + kBlockX - 1) / kBlockX, (nz + kBlockZ - 1) / kBlockZ, (ny + kNJ - 1) / kNJ);
-    hipLaunchKernelGGL((bp_column_kernel<TP, FDK, kNJ>), grid, block, 0, s, dcoef, nviews, p, nu, nv, vin, vout, nx,
+    dim3 grid((nx + 15) / 16, (nz + 15) / 16, (ny + kNJ - 1) / kNJ);
+    hipLaunchKernelGGL((bp_lds_kernel<TP, FDK, kNJ>), grid, dim3(256), 0, s, dcoef, nviews, p, nu, nv, vin, vout, nx,
                        ny, nz, ci, cj, ck);
   }
   else
