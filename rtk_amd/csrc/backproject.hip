@@ -107,7 +107,34 @@ Footprint(const ViewCoef & c, float xlo, float xhi, float ylo, float yhi, float 
 // buffer after it, so there is one barrier per view and the loads overlap the
 // arithmetic.  Taps outside the staged rectangle (rounding) and views whose
 // footprint does not fit read global memory; results are identical either way.
-template <typename TP, bool FDK, int NJ>
+// Four consecutive pixels from an aligned address, as FP32.
+__device__ __forceinline__ void
+Load4(const float * p, float * d)
+{
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const f4 f = *reinterpret_cast<const f4 *>(p);
+  d[0] = f.x;
+  d[1] = f.y;
+  d[2] = f.z;
+  d[3] = f.w;
+}
+template <typename TP>
+__device__ __forceinline__ void
+Load4(const TP * p, float * d)
+{
+  typedef unsigned short u4 __attribute__((ext_vector_type(4)));
+  const u4 f = *reinterpret_cast<const u4 *>(p);
+  const unsigned short e[4] = { f.x, f.y, f.z, f.w };
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+  {
+    TP v;
+    __builtin_memcpy(&v, &e[q], 2);
+    d[q] = LoadPix(&v, 0);
+  }
+}
+
+template <typename TP, bool FDK, int NJ, bool VEC>
 __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict__ views, int nviews,
                                                      const TP * __restrict__ proj, int nu, int nv,
                                                      const float * vol_in, float * vol_out, int nx, int ny, int nz,
@@ -123,10 +150,13 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
   const long       sliceStride = long(nx) * ny;
   const long       base = long(k) * sliceStride + long(j0) * nx + i;
   const int        jn = min(NJ, ny - j0);
+  // the views are summed from zero and vol_in is added at the end: no global load is
+  // outstanding when the view loop starts, so the compiler's wait for the staging
+  // loads stays where their data is stored
   float            acc[NJ];
 #pragma unroll
   for (int jj = 0; jj < NJ; jj++)
-    acc[jj] = (live && jj < jn && vol_in) ? vol_in[base + long(jj) * nx] : 0.f;
+    acc[jj] = 0.f;
 
   const float x = float(i) - ci, z = float(k) - ck, y0 = float(j0) - cj;
   const float xlo = float(bx0) - ci, xhi = float(min(bx0 + 15, nx - 1)) - ci;
@@ -137,41 +167,37 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
   // staging: lane = 4 consecutive pixels (one 16-byte load for FP32 when rows are
   // 4-aligned), 16 lanes per 64-pixel row, 16 rows per pass, 4 passes
   const int   sc = 4 * (t & 15), sr = t >> 4;
-  const bool  vec = (nu & 3) == 0 && (reinterpret_cast<size_t>(proj) & 15) == 0;
 
   float sreg[16];
   int   cu0, cv0, cW, cH; // footprint of the view being backprojected
+  // VEC (nu a multiple of 4, 16-byte aligned stack): every lane loads its four 4-pixel
+  // chunks, 16 bytes (8 for 16-bit stacks) each, with no branch, so the loaded
+  // registers are never copied before the arithmetic that hides their latency.  With
+  // nu, u0 and sc multiples of 4, a chunk that starts inside the rectangle (sc < W)
+  // lies inside its detector row; lanes outside read the view's first 4 pixels
+  // instead, which store() discards with the same test.
   auto  load = [&](int p, int u0, int v0, int W, int H) {
-    const long o0 = long(p) * projStride + long(v0 + sr) * nu + (u0 + sc);
-#pragma unroll
-    for (int q = 0; q < 4; q++)
+    const long vbase = long(p) * projStride;
+    if constexpr (VEC)
     {
-      const int  r = sr + 16 * q;
-      const long o = o0 + long(16 * q) * nu;
-      if (vec && r < H && sc < W && u0 + sc + 3 <= nu - 1)
-      {
-        if constexpr (std::is_same<TP, float>::value)
-        {
-          typedef float f4 __attribute__((ext_vector_type(4)));
-          const f4 f = *reinterpret_cast<const f4 *>(proj + o);
-          sreg[4 * q] = f.x;
-          sreg[4 * q + 1] = f.y;
-          sreg[4 * q + 2] = f.z;
-          sreg[4 * q + 3] = f.w;
-        }
-        else
-        {
 #pragma unroll
-          for (int e = 0; e < 4; e++)
-            sreg[4 * q + e] = LoadPix(proj, o + e);
-        }
-      }
-      else
+      for (int q = 0; q < 4; q++)
       {
+        const int  r = sr + 16 * q;
+        const long o = (r < H && sc < W) ? vbase + long(v0 + r) * nu + (u0 + sc) : vbase;
+        Load4(proj + o, sreg + 4 * q);
+      }
+    }
+    else
+    {
+#pragma unroll
+      for (int q = 0; q < 4; q++)
 #pragma unroll
         for (int e = 0; e < 4; e++)
-          sreg[4 * q + e] = (r < H && sc + e < W) ? LoadPix(proj, o + e) : 0.f;
-      }
+        {
+          const int r = sr + 16 * q;
+          sreg[4 * q + e] = (r < H && sc + e < W) ? LoadPix(proj, vbase + long(v0 + r) * nu + (u0 + sc + e)) : 0.f;
+        }
     }
   };
   // whole 4-pixel chunks are stored: cells at or beyond W hold pixels (or zeros) that
@@ -208,11 +234,8 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
   cv0 = rects[0][1];
   cW = rects[0][2];
   cH = rects[0][3];
-  if (cW > 0)
-  {
-    load(0, cu0, cv0, cW, cH);
-    store(0, cW, cH);
-  }
+  load(0, cu0, cv0, cW, cH);
+  store(0, cW, cH);
   __syncthreads();
   for (int p = 0; p < nviews; p++)
   {
@@ -224,8 +247,7 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
       nv0 = rects[(p + 1) & 1][1];
       nW = rects[(p + 1) & 1][2];
       nH = rects[(p + 1) & 1][3];
-      if (nW > 0)
-        load(p + 1, nu0, nv0, nW, nH);
+      load(p + 1, nu0, nv0, nW, nH);
     }
     if (t == 0 && p + 2 < nviews)
       publish(p + 2);
@@ -263,10 +285,11 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
             const float   vv = vb + float(jj) * dv;
             const float   vf = floorf(vv);
             const float   fv = vv - vf;
-            const float * s0 = S + (int(vf) * kTileStride + off);
+            // 24-bit multiply: the LDS offset is small and the 32 x 32 form is slower
+            const float * s0 = S + (int(__umul24(unsigned(int(vf)), unsigned(kTileStride))) + off);
             const float   t0 = s0[0] + fv * (s0[kTileStride] - s0[0]);
             const float   t1 = s0[1] + fv * (s0[kTileStride + 1] - s0[1]);
-            acc[jj] += w0 * t0 + w1 * t1;
+            acc[jj] = fmaf(w1, t1, fmaf(w0, t0, acc[jj]));
           }
         }
         else
@@ -304,7 +327,7 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
         }
       }
     }
-    if (nW > 0)
+    if (p + 1 < nviews)
       store(b ^ 1, nW, nH);
     __syncthreads();
     cu0 = nu0;
@@ -317,7 +340,7 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
 #pragma unroll
     for (int jj = 0; jj < NJ; jj++)
       if (jj < jn)
-        vol_out[base + long(jj) * nx] = acc[jj];
+        vol_out[base + long(jj) * nx] = (vol_in ? vol_in[base + long(jj) * nx] : 0.f) + acc[jj];
   }
 }
 
@@ -418,9 +441,14 @@ Launch(bool column, const ViewCoef * dcoef, const int * dborder, int nviews, con
   const TP * p = static_cast<const TP *>(proj);
   if (column)
   {
-    dim3 grid((nx + 15) / 16, (nz + 15) / 16, (ny + kNJ - 1) / kNJ);
-    hipLaunchKernelGGL((bp_lds_kernel<TP, FDK, kNJ>), grid, dim3(256), 0, s, dcoef, nviews, p, nu, nv, vin, vout, nx,
-                       ny, nz, ci, cj, ck);
+    dim3       grid((nx + 15) / 16, (nz + 15) / 16, (ny + kNJ - 1) / kNJ);
+    const bool vec = (nu & 3) == 0 && (reinterpret_cast<size_t>(p) & 15) == 0;
+    if (vec)
+      hipLaunchKernelGGL((bp_lds_kernel<TP, FDK, kNJ, true>), grid, dim3(256), 0, s, dcoef, nviews, p, nu, nv, vin,
+                         vout, nx, ny, nz, ci, cj, ck);
+    else
+      hipLaunchKernelGGL((bp_lds_kernel<TP, FDK, kNJ, false>), grid, dim3(256), 0, s, dcoef, nviews, p, nu, nv, vin,
+                         vout, nx, ny, nz, ci, cj, ck);
   }
   else
   {
