@@ -8,7 +8,7 @@
 //
 // Design (CDNA4): one workgroup filters one detector row entirely in LDS.  The
 // zero-padded real row of length N is packed into N/2 complex points
-// (even + i odd), transformed by a radix-2 FFT in LDS, split into the N/2+1
+// (even + i odd), transformed by a radix-4/2 FFT in LDS, split into the N/2+1
 // half spectrum, multiplied by the kernel, re-packed and inverse transformed;
 // only the cropped centre is written.  The weighting is fused into the load
 // and the output may be written as FP16/BF16 for the backprojector, so the
@@ -93,28 +93,59 @@ StoreOut(__hip_bfloat16 * p, long i, float v)
   p[i] = __float2bfloat16(v);
 }
 
-// In-place radix-2 DIT FFT of M points in LDS whose input is stored in
-// bit-reversed order; tw[t] = exp(-2 pi i t / M).  INV uses conj(tw).
+// In-place DIT FFT of M points in LDS whose input is stored in bit-reversed order;
+// tw[t] = exp(-2 pi i t / M) for t < 3M/4.  INV uses conj(tw).  Pairs of radix-2
+// stages are fused into radix-4 stages (one barrier and one LDS pass per two
+// stages); an odd log2(M) starts with one radix-2 stage.  With L = 2^s and
+// w = W_{4L}^p, the four points x0..x3 at p, p+L, p+2L, p+3L of a group become
+//   a0 = x0, a1 = w^2 x1, a2 = w x2, a3 = w^3 x3,
+//   y0 = (a0 + a1) + (a2 + a3), y2 = (a0 + a1) - (a2 + a3),
+//   y1 = (a0 - a1) -/+ i (a2 - a3), y3 = (a0 - a1) +/- i (a2 - a3)  (forward/inverse),
+// which is two radix-2 stages (twiddles W_{2L}^p = w^2, then w and w W_{4L}^L).
 template <bool INV>
 __device__ __forceinline__ void
 LdsFFT(float2 * buf, const float2 * tw, int M, int logM)
 {
-  for (int s = 0; s < logM; s++)
+  int s = 0;
+  if (logM & 1)
   {
     __syncthreads();
-    const int half = 1 << s;
     for (int b = threadIdx.x; b < (M >> 1); b += blockDim.x)
     {
-      const int pos = b & (half - 1);
-      const int i0 = ((b >> s) << (s + 1)) + pos;
-      const int i1 = i0 + half;
-      float2    w = tw[pos << (logM - 1 - s)];
+      const float2 a = buf[2 * b], c = buf[2 * b + 1];
+      buf[2 * b] = make_float2(a.x + c.x, a.y + c.y);
+      buf[2 * b + 1] = make_float2(a.x - c.x, a.y - c.y);
+    }
+    s = 1;
+  }
+  for (; s < logM; s += 2)
+  {
+    __syncthreads();
+    const int L = 1 << s;
+    const int shift = logM - 2 - s; // twiddle index step M / (4L) = 2^shift
+    for (int b = threadIdx.x; b < (M >> 2); b += blockDim.x)
+    {
+      const int pos = b & (L - 1);
+      const int i0 = ((b >> s) << (s + 2)) + pos;
+      float2    w1 = tw[pos << shift], w2 = tw[(2 * pos) << shift], w3 = tw[(3 * pos) << shift];
       if (INV)
-        w.y = -w.y;
-      const float2 a = buf[i0];
-      const float2 c = cmul(buf[i1], w);
-      buf[i0] = make_float2(a.x + c.x, a.y + c.y);
-      buf[i1] = make_float2(a.x - c.x, a.y - c.y);
+      {
+        w1.y = -w1.y;
+        w2.y = -w2.y;
+        w3.y = -w3.y;
+      }
+      const float2 a0 = buf[i0];
+      const float2 a1 = cmul(buf[i0 + L], w2);
+      const float2 a2 = cmul(buf[i0 + 2 * L], w1);
+      const float2 a3 = cmul(buf[i0 + 3 * L], w3);
+      const float2 s01 = make_float2(a0.x + a1.x, a0.y + a1.y), d01 = make_float2(a0.x - a1.x, a0.y - a1.y);
+      const float2 s23 = make_float2(a2.x + a3.x, a2.y + a3.y), d23 = make_float2(a2.x - a3.x, a2.y - a3.y);
+      // -i d = (d.y, -d.x); +i d = (-d.y, d.x)
+      const float2 r = INV ? make_float2(-d23.y, d23.x) : make_float2(d23.y, -d23.x);
+      buf[i0] = make_float2(s01.x + s23.x, s01.y + s23.y);
+      buf[i0 + L] = make_float2(d01.x + r.x, d01.y + r.y);
+      buf[i0 + 2 * L] = make_float2(s01.x - s23.x, s01.y - s23.y);
+      buf[i0 + 3 * L] = make_float2(d01.x - r.x, d01.y - r.y);
     }
   }
   __syncthreads();
@@ -126,8 +157,8 @@ BitRev(int m, int logM)
   return int(__brev(unsigned(m)) >> (32 - logM));
 }
 
-// One workgroup per (row, view).  tables: [0, M/2) FFT twiddles exp(-2 pi i t/M),
-// [M/2, M/2+M+1) split twiddles exp(-2 pi i k/N), then kernel K[k] for k in [0, M].
+// One workgroup per (row, view).  tables: [0, 3M/4) FFT twiddles exp(-2 pi i t/M),
+// [3M/4, 3M/4+M+1) split twiddles exp(-2 pi i k/N), then kernel K[k] for k in [0, M].
 template <typename TO>
 __global__ __launch_bounds__(256) void ramp_kernel(const float * __restrict__ in, TO * __restrict__ out, int nu,
                                                     int nv, int N, int logM, int zeroext,
@@ -139,12 +170,13 @@ __global__ __launch_bounds__(256) void ramp_kernel(const float * __restrict__ in
   const int M = N >> 1;
   float2 *  buf = lds;          // M: forward transform
   float2 *  buf2 = lds + M;     // M: inverse transform
-  float2 *  tw = lds + 2 * M;   // M/2
+  float2 *  tw = lds + 2 * M;   // 3M/4
   const int j = blockIdx.x;
   const int p = blockIdx.y;
   const float * row = in + (long(p) * nv + j) * nu;
   TO *          orow = out + (long(p) * nv + j) * nu;
-  for (int t = threadIdx.x; t < (M >> 1); t += blockDim.x)
+  const int ntw = (3 * M) >> 2;
+  for (int t = threadIdx.x; t < ntw; t += blockDim.x)
     tw[t] = tables[t];
   const WeightView * wv = wviews ? wviews + p : nullptr;
   auto               W = [&](int i) -> float { return wv ? row[i] * CosGammaWeight(*wv, dx, dy, i, j) : row[i]; };
@@ -170,7 +202,7 @@ __global__ __launch_bounds__(256) void ramp_kernel(const float * __restrict__ in
 
   // Split to the half spectrum, multiply by the kernel, re-pack for the inverse
   // into the second buffer (bit-reversed for the inverse DIT FFT).
-  const float2 * split = tables + (M >> 1);
+  const float2 * split = tables + ntw;
   const float2 * K = split + (M + 1);
   const int      npairs = (M >> 1) + 1; // k = 0 .. M/2, partner M-k
   for (int k = threadIdx.x; k < npairs; k += blockDim.x)
@@ -317,13 +349,14 @@ rtkhip_ramp_filter(const int proj_dim[3], int xpad, const double * kernel, int t
     const long zeroext = (long(xpad) - nu) / 2;
     RTKHIP_REQUIRE(zeroext >= trunc_next, RTKHIP_ERR_INVALID_ARGUMENT, "ramp_filter: mirror wider than padding");
     const double pi = 3.141592653589793238462643383279502884;
-    std::vector<float2> tab(M / 2 + (M + 1) + (M + 1));
-    for (int t = 0; t < M / 2; t++)
+    const int           ntw = (3 * M) / 4;
+    std::vector<float2> tab(ntw + (M + 1) + (M + 1));
+    for (int t = 0; t < ntw; t++)
       tab[t] = make_float2(float(std::cos(-2 * pi * t / M)), float(std::sin(-2 * pi * t / M)));
     for (int k = 0; k <= M; k++)
-      tab[M / 2 + k] = make_float2(float(std::cos(-2 * pi * k / xpad)), float(std::sin(-2 * pi * k / xpad)));
+      tab[ntw + k] = make_float2(float(std::cos(-2 * pi * k / xpad)), float(std::sin(-2 * pi * k / xpad)));
     for (int k = 0; k <= M; k++)
-      tab[M / 2 + M + 1 + k] = make_float2(float(kernel[2 * k]), float(kernel[2 * k + 1]));
+      tab[ntw + M + 1 + k] = make_float2(float(kernel[2 * k]), float(kernel[2 * k + 1]));
     auto & dt = g_ramp.tables[s];
     dt.Resize(tab.size());
     RTKHIP_CHECK(hipMemcpyAsync(dt.get(), tab.data(), sizeof(float2) * tab.size(), hipMemcpyHostToDevice, s));
@@ -350,7 +383,7 @@ rtkhip_ramp_filter(const int proj_dim[3], int xpad, const double * kernel, int t
       dx = float(proj_spacing[0]);
       dy = float(proj_spacing[1]);
     }
-    const size_t lds = sizeof(float2) * (2 * M + M / 2);
+    const size_t lds = sizeof(float2) * (2 * M + (3 * M) / 4);
     int          dev = 0, maxLds = 0;
     RTKHIP_CHECK(hipGetDevice(&dev));
     RTKHIP_CHECK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
