@@ -304,13 +304,16 @@ struct Walker
     ciy = min(max(iy, 0), R.nym2);
     lx = float(fx);
     ly = float(fy);
+    // fraction in [0, 1) plus a step of magnitude <= 1 lies in [-1, 2): its floor is
+    // -1, 0 or 1 and subtracting it is exact (the index clamp above keeps even a
+    // non-finite fraction in bounds)
     fx += R.stepx;
     fy += R.stepy;
-    const bool xu = fx >= 1., xd = fx < 0., yu = fy >= 1., yd = fy < 0.;
-    fx = xu ? fx - 1. : (xd ? fx + 1. : fx);
-    fy = yu ? fy - 1. : (yd ? fy + 1. : fy);
-    ix += xu ? 1 : (xd ? -1 : 0);
-    iy += yu ? 1 : (yd ? -1 : 0);
+    const double flx = floor(fx), fly = floor(fy);
+    fx -= flx;
+    fy -= fly;
+    ix += int(flx);
+    iy += int(fly);
   }
 };
 
@@ -322,54 +325,116 @@ __global__ __launch_bounds__(256) void joseph_kernel(const JosephView * __restri
                                                       double spx, double spy, double spz, double infClip,
                                                       double supClip, unsigned long long * samples)
 {
-  const int iu = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int jv = blockIdx.y * 16 + (threadIdx.x >> 4);
-  const int p = blockIdx.z;
-  long long cnt = 0;
-  if (iu < nu && jv < nv)
+  const int  iu = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int  jv = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int  p = blockIdx.z;
+  const bool inside = iu < nu && jv < nv;
+  const long pix = (long(p) * nv + jv) * nu + iu;
+  long long  cnt = 0;
+  Ray        R;
+  const bool hit = inside && SetupRay<false>(views[p], iu, jv, p, nx, ny, nz, spx, spy, spz, infClip, supClip, R);
+  auto       border = [&](double step, const float * bb, double x, double y) -> float {
+    int   t[4];
+    float w[4];
+    BorderTaps(R, x, y, t, w);
+    float r = 0.f;
+    r += w[0] * bb[t[0]];
+    r += w[1] * bb[t[1]];
+    r += w[2] * bb[t[2]];
+    r += w[3] * bb[t[3]];
+    return float(double(r) * step);
+  };
+  auto middle = [&](Walker & wk, const float * bb) -> float {
+    float       lx, ly;
+    const int   idx = wk.Stage(R, lx, ly);
+    const float lxc = 1.f - lx, lyc = 1.f - ly;
+    return (lxc * lyc) * bb[idx] + (lx * lyc) * bb[idx + 1] + (lxc * ly) * bb[idx + R.oy] + (lx * ly) * bb[idx + 1 + R.oy];
+  };
+  float sum = 0.f;
+
+  // The wavefront's rays that share its most common dominant axis and walking
+  // direction step through absolute slice indices together, so at every step the
+  // 64 lanes read neighbouring voxels of one slice (a ray-relative step count would
+  // put lanes that entered the volume at different slices a slice stride apart).
+  // Each ray still sums its own samples in its own order.
+  const int key = hit ? 2 * R.axis + (R.oz < 0 ? 1 : 0) : -1;
+  int       best = 0, bestCount = -1;
+  for (int q = 0; q < 6; q++)
   {
-    const long pix = (long(p) * nv + jv) * nu + iu;
-    Ray        R;
-    if (SetupRay<false>(views[p], iu, jv, p, nx, ny, nz, spx, spy, spz, infClip, supClip, R))
+    const int c = __popcll(__ballot(key == q));
+    if (c > bestCount)
     {
-      const float * b = (R.transposed ? volT : vol) + R.start;
-      auto          border = [&](double step, const float * bb, double x, double y) -> float {
-        int   t[4];
-        float w[4];
-        BorderTaps(R, x, y, t, w);
-        float r = 0.f;
-        r += w[0] * bb[t[0]];
-        r += w[1] * bb[t[1]];
-        r += w[2] * bb[t[2]];
-        r += w[3] * bb[t[3]];
-        return float(double(r) * step);
-      };
-      float sum = 0.f;
-      if (R.fs == R.ns)
-        sum += border(R.wOne, b, R.cx, R.cy);
-      else
-      {
-        sum += border(R.wB, b, R.cx, R.cy);
-        b += R.oz;
-        Walker wk;
-        wk.Init(R.cx + R.stepx, R.cy + R.stepy);
-        for (int t = 0; t < R.nmid; ++t)
-        {
-          float       lx, ly;
-          const int   idx = wk.Stage(R, lx, ly);
-          const float lxc = 1.f - lx, lyc = 1.f - ly;
-          sum += (lxc * lyc) * b[idx] + (lx * lyc) * b[idx + 1] + (lxc * ly) * b[idx + R.oy] +
-                 (lx * ly) * b[idx + 1 + R.oy];
-          b += R.oz;
-        }
-        sum += border(R.wE, b, wk.ix + wk.fx, wk.iy + wk.fy);
-      }
-      cnt = abs(R.fs - R.ns) + 1;
-      pout[pix] = float(double(pin ? pin[pix] : 0.f) + double(sum) * R.stepNorm);
+      best = q;
+      bestCount = c;
     }
-    else
-      pout[pix] = pin ? pin[pix] : 0.f;
   }
+  const bool lock = hit && key == best;
+  int        klo = lock ? min(R.ns, R.fs) : INT_MAX, khi = lock ? max(R.ns, R.fs) : INT_MIN;
+  for (int o = 32; o > 0; o >>= 1)
+  {
+    klo = min(klo, __shfl_xor(klo, o, 64));
+    khi = max(khi, __shfl_xor(khi, o, 64));
+  }
+  if (klo <= khi)
+  {
+    const int     axis = best >> 1, dirSign = (best & 1) ? -1 : 1;
+    const float * buf = axis == 0 ? volT : vol;
+    const long    absOz = axis == 0 ? long(ny) * nz : (axis == 1 ? long(nx) : long(nx) * ny);
+    const int     kfirst = dirSign > 0 ? klo : khi;
+    const int     nsl = khi - klo + 1;
+    const int     t0 = lock ? (R.ns - kfirst) * dirSign : INT_MAX;
+    const int     tEnd = lock ? t0 + R.nmid + 1 : INT_MIN; // R.nmid = -1 when fs == ns
+    Walker        wk;
+    for (int s = 0; s < nsl; ++s)
+    {
+      if (s >= t0 && s <= tEnd)
+      {
+        const float * bb = buf + (kfirst + s * dirSign) * absOz;
+        const int     t = s - t0;
+        if (t == 0)
+        {
+          if (R.fs == R.ns)
+            sum += border(R.wOne, bb, R.cx, R.cy);
+          else
+          {
+            sum += border(R.wB, bb, R.cx, R.cy);
+            wk.Init(R.cx + R.stepx, R.cy + R.stepy);
+          }
+        }
+        else if (t <= R.nmid)
+          sum += middle(wk, bb);
+        else
+          sum += border(R.wE, bb, wk.ix + wk.fx, wk.iy + wk.fy);
+      }
+    }
+  }
+  if (hit && !lock)
+  {
+    // rays of a minority axis or direction: their own walk
+    const float * b = (R.transposed ? volT : vol) + R.start;
+    if (R.fs == R.ns)
+      sum += border(R.wOne, b, R.cx, R.cy);
+    else
+    {
+      sum += border(R.wB, b, R.cx, R.cy);
+      b += R.oz;
+      Walker wk;
+      wk.Init(R.cx + R.stepx, R.cy + R.stepy);
+      for (int t = 0; t < R.nmid; ++t)
+      {
+        sum += middle(wk, b);
+        b += R.oz;
+      }
+      sum += border(R.wE, b, wk.ix + wk.fx, wk.iy + wk.fy);
+    }
+  }
+  if (hit)
+  {
+    cnt = abs(R.fs - R.ns) + 1;
+    pout[pix] = float(double(pin ? pin[pix] : 0.f) + double(sum) * R.stepNorm);
+  }
+  else if (inside)
+    pout[pix] = pin ? pin[pix] : 0.f;
   if (COUNT)
   {
     // Wavefront reduction, then one atomic per wavefront.
