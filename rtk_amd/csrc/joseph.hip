@@ -507,6 +507,18 @@ SplatRay(const Ray & R, float value, float * acc, float * accT)
 // window, one cell per lane.  Taps that still fall outside (rounding), rays of a
 // minority axis or direction, and tiles whose footprint exceeds the window or that
 // hold a non-finite value take the per-ray path.
+// round(v * scale) as a two's-complement 64-bit integer, for |v * scale| < 2^51:
+// adding 1.5 * 2^52 leaves the rounded integer (round to nearest even, as
+// __double2ll_rn) in the low mantissa bits, and subtracting the constant's bit
+// pattern recovers it, in one FMA and one 64-bit subtraction.
+__device__ __forceinline__ unsigned long long
+ToFixed(float v, double scale)
+{
+  constexpr double kMagic = 6755399441055744.0; // 1.5 * 2^52
+  const double     t = fma(double(v), scale, kMagic);
+  return (unsigned long long)(__double_as_longlong(t) - __double_as_longlong(kMagic));
+}
+
 constexpr int kTileWin = 32;    // LDS window edge, in voxels
 constexpr int kTileStride = 33; // LDS row stride, in cells
 
@@ -523,6 +535,7 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
   __shared__ int    krange[2]; // min and max slice over the tile's participating rays
   __shared__ float  line[4][4]; // corner c: in-slice x = line[c][0] + k line[c][1], y likewise
   __shared__ int    cornerOk;
+  __shared__ int    wpar[2][3]; // window origin and edge of the next two slices, by thread 0
 
   const int tid = threadIdx.x;
   const int u0 = blockIdx.x * 16, v0 = blockIdx.y * 16, p = blockIdx.z;
@@ -649,16 +662,29 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
   Walker       wk;
   int          pwx = 0, pwy = 0; // window origin of the previous slice
   int          plim = kTileWin;  // cells per window edge flushed for the previous slice
+  // thread 0 places the window of slice s + 1 while slice s is processed, in
+  // wpar[(s + 1) & 1]; that slot was last read before the barrier ending slice s - 1.
+  // The window edge is 16 when the slice footprint fits.
+  auto place = [&](int k, int * out) {
+    float sx, sy;
+    window(k, out[0], out[1], sx, sy);
+    out[2] = (sx + 5.f <= 16.f && sy + 5.f <= 16.f) ? 16 : kTileWin;
+  };
+  if (tid == 0)
+    place(kfirst, wpar[0]);
+  __syncthreads();
   for (int s = 0; s <= nsl; ++s)
   {
     const int k = kfirst + s * dirSign;
     int       wx = 0, wy = 0;
-    int       lim = kTileWin; // window edge used for this slice: 16 when the footprint fits
+    int       lim = kTileWin; // window edge used for this slice
     if (s < nsl)
     {
-      float sx, sy;
-      window(k, wx, wy, sx, sy);
-      lim = (sx + 5.f <= 16.f && sy + 5.f <= 16.f) ? 16 : kTileWin;
+      wx = wpar[s & 1][0];
+      wy = wpar[s & 1][1];
+      lim = wpar[s & 1][2];
+      if (tid == 0 && s + 1 < nsl)
+        place(k + dirSign, wpar[(s + 1) & 1]);
       if (s >= r0 && s <= r1)
       {
         unsigned long long * w = win[s & 1];
@@ -712,7 +738,7 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
           const int   a = tx[q] - wx, b = ty[q] - wy;
           const float v = sc * tw[q];
           if (unsigned(a) < unsigned(lim) && unsigned(b) < unsigned(lim))
-            atomicAdd(&w[b * kTileStride + a], (unsigned long long)__double2ll_rn(double(v) * toFixed));
+            atomicAdd(&w[b * kTileStride + a], ToFixed(v, toFixed));
           else
             unsafeAtomicAdd(g + tx[q] + ty[q] * oy, v);
         }
