@@ -138,15 +138,18 @@ def cg_measurement(rtk, np, torch, max_iter, tol):
     proj = clean.like((-torch.log(torch.clamp(counts, min=1.) / 1e4)).float().contiguous())
     vol = rtk.ConstantImageSource((nx, nx, nx), (1., 1., 1.), (-(nx - 1) / 2.,) * 3)
     out = {}
-    runs = [("fixed_30_voxel_bp", 30, 0., None), ("to_tol_voxel_bp", max_iter, tol, None),
-            ("to_tol_joseph_bp", max_iter, tol, "joseph")]
-    for name, niter, t, bp in runs:
+    runs = [("fixed_30_voxel_bp", 30, 0., None, "f32"), ("fixed_30_voxel_bp_f16_stack", 30, 0., None, "f16"),
+            ("fixed_30_voxel_bp_bf16_stack", 30, 0., None, "bf16"), ("to_tol_voxel_bp", max_iter, tol, None, "f32"),
+            ("to_tol_joseph_bp", max_iter, tol, "joseph", "f32")]
+    ref = None
+    for name, niter, t, bp, dtype in runs:
         cg = rtk.HipConjugateGradientConeBeamReconstructionFilter()
         cg.SetInput(0, vol)
         cg.SetInput(1, proj)
         cg.SetGeometry(g)
         if bp == "joseph":
             cg.SetBackProjectionFilter(cg.BP_JOSEPH)
+        cg.SetStackDtype(dtype)
         cg.SetNumberOfIterations(niter)
         cg.SetResidualTolerance(t)
         torch.cuda.synchronize()
@@ -160,8 +163,15 @@ def cg_measurement(rtk, np, torch, max_iter, tol):
                      "ms_per_iteration": round(dt * 1e3 / max(1, it), 2)}
         if t:
             out[name]["reached"] = bool(res[-1] <= t)
+        x = cg.GetOutput().tensor
+        if name == "fixed_30_voxel_bp":
+            ref = x.double()
+        elif dtype != "f32":
+            # the 16-bit stack's own error: relative L2 against the FP32-stack run
+            out[name]["rel_l2_vs_f32_stack"] = float(torch.linalg.norm(x.double() - ref) / torch.linalg.norm(ref))
         log(f"cg {name}: {out[name]}")
-    return {"workload": "CG 256^3 @1 mm <- 360 x 512^2 @0.8 mm, Poisson I0=1e4, zero start",
+    return {"workload": "CG 256^3 @1 mm <- 360 x 512^2 @0.8 mm, Poisson I0=1e4, zero start; FP32/FP16/BF16 stacks "
+                        "read by the backprojector (FP32 arithmetic, FP64 CG scalars)",
             "target_rel_residual": tol, "max_iterations": max_iter, **out}
 
 

@@ -252,7 +252,10 @@ enum rtkhip_backprojector
  *                norm is the CG recursion's R_k, FP64
  *   stack_dtype  storage of the stacks the backprojector reads (enum rtkhip_dtype): the
  *                weighted measured projections and each iteration's W FP(P) are
- *                rounded to FP16/BF16 once, all arithmetic stays FP32/FP64
+ *                rounded to FP16/BF16 once, all arithmetic stays FP32/FP64; FP16
+ *                stacks are scaled by a power of two that maps their max |value|
+ *                into (2^13, 2^14] (W FP(P) grows with the unnormalised direction P),
+ *                undone exactly after the backprojection
  *   resid        optional host array of niter+1 doubles: ||R_k|| / ||B||
  *   iterations   optional: iterations done */
 int rtkhip_cg_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const rtkhip_image * vol, float * dev_x,

@@ -281,7 +281,8 @@ struct rtkhip_context
 {
   rtkhip::DeviceBuffer<unsigned char> filtered;
   rtkhip::DeviceBuffer<float>         vB, vR, vP, vAP, pT;
-  rtkhip::DeviceBuffer<unsigned short> pT16; // 16-bit stack read by the CG backprojector
+  rtkhip::DeviceBuffer<unsigned short> pT16;   // 16-bit stack read by the CG backprojector
+  rtkhip::DeviceBuffer<unsigned>       maxBits; // max |stack| for the FP16 scaling
   rtkhip::DeviceBuffer<double>        state;
   struct RampKey
   {
@@ -547,13 +548,57 @@ Multiply(size_t n, const float * a, const float * b, float * out, hipStream_t s)
   RTKHIP_CHECK_LAUNCH();
 }
 
-// out = TO(a * w) (w NULL = ones): the weighted stack in its 16-bit storage type.
-template <typename TO>
-__global__ __launch_bounds__(256) void weight_store_kernel(size_t n, const float * a, const float * w, TO * out)
+// Power-of-two factor that brings max|stack| (as FP32 bits) into (2^13, 2^14]: FP16
+// holds values up to 65504, and the CG operator's stacks W FP(P) grow with the
+// unnormalised search direction P.  Zero maps to 1; Inf/NaN propagate.
+__device__ __forceinline__ int
+Fp16ScaleExponent(unsigned maxBits)
 {
+  const float m = __uint_as_float(maxBits);
+  if (m == 0.f)
+    return 0;
+  int e;
+  frexpf(m, &e); // m < 2^e
+  return 14 - max(e, -100);
+}
+
+// Clears the max word with a kernel rather than a memset node, so a caller may
+// capture the sequence in a graph.
+__global__ void clear_word_kernel(unsigned * w)
+{
+  *w = 0u;
+}
+
+// *maxBits = max |a * w| over the stack (w NULL = ones), as FP32 bits
+// (non-negative floats order like their bit patterns).
+__global__ __launch_bounds__(256) void absmax_kernel(size_t n, const float * a, const float * w, unsigned * maxBits)
+{
+  __shared__ unsigned red[256];
+  unsigned            m = 0;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    m = max(m, __float_as_uint(fabsf(w ? a[i] * w[i] : a[i])));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1)
+  {
+    if (int(threadIdx.x) < o)
+      red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    atomicMax(maxBits, red[0]);
+}
+
+// out = TO(a * w * 2^k) (w NULL = ones; k from maxBits, 0 when maxBits is NULL):
+// the weighted stack in its 16-bit storage type.
+template <typename TO>
+__global__ __launch_bounds__(256) void weight_store_kernel(size_t n, const float * a, const float * w,
+                                                            const unsigned * maxBits, TO * out)
+{
+  const float scale = maxBits ? ldexpf(1.f, Fp16ScaleExponent(*maxBits)) : 1.f;
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
   {
-    const float v = w ? a[i] * w[i] : a[i];
+    const float v = (w ? a[i] * w[i] : a[i]) * scale;
     if constexpr (std::is_same<TO, __half>::value)
       out[i] = __float2half(v);
     else
@@ -561,16 +606,40 @@ __global__ __launch_bounds__(256) void weight_store_kernel(size_t n, const float
   }
 }
 
+// vol *= 2^-k: undoes the stack scaling after the (linear) backprojection.
+__global__ __launch_bounds__(256) void unscale_kernel(size_t n, float * vol, const unsigned * maxBits)
+{
+  const float inv = ldexpf(1.f, -Fp16ScaleExponent(*maxBits));
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    vol[i] *= inv;
+}
+
+// Stores W a in the 16-bit stack; for FP16 also computes the scaling (maxBits must
+// then be a device word) that Unscale16 removes after the backprojection.
 void
-WeightAndStore(size_t n, const float * a, const float * w, void * out, int dtype, hipStream_t s)
+WeightAndStore(size_t n, const float * a, const float * w, void * out, int dtype, unsigned * maxBits, hipStream_t s)
 {
   const unsigned g = unsigned(std::min<size_t>((n + 255) / 256, 4096));
   if (dtype == RTKHIP_F16)
+  {
+    hipLaunchKernelGGL(clear_word_kernel, dim3(1), dim3(1), 0, s, maxBits);
+    RTKHIP_CHECK_LAUNCH();
+    hipLaunchKernelGGL(absmax_kernel, dim3(g ? g : 1), dim3(256), 0, s, n, a, w, maxBits);
+    RTKHIP_CHECK_LAUNCH();
     hipLaunchKernelGGL(weight_store_kernel<__half>, dim3(g ? g : 1), dim3(256), 0, s, n, a, w,
-                       static_cast<__half *>(out));
+                       static_cast<const unsigned *>(maxBits), static_cast<__half *>(out));
+  }
   else
     hipLaunchKernelGGL(weight_store_kernel<__hip_bfloat16>, dim3(g ? g : 1), dim3(256), 0, s, n, a, w,
-                       static_cast<__hip_bfloat16 *>(out));
+                       static_cast<const unsigned *>(nullptr), static_cast<__hip_bfloat16 *>(out));
+  RTKHIP_CHECK_LAUNCH();
+}
+
+void
+Unscale16(size_t n, float * vol, const unsigned * maxBits, hipStream_t s)
+{
+  const unsigned g = unsigned(std::min<size_t>((n + 255) / 256, 4096));
+  hipLaunchKernelGGL(unscale_kernel, dim3(g ? g : 1), dim3(256), 0, s, n, vol, maxBits);
   RTKHIP_CHECK_LAUNCH();
 }
 
@@ -889,7 +958,10 @@ rtkhip_cg_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const rtkhi
     c->vAP.Resize(nvox);
     c->pT.Resize(npix);
     if (stack_dtype != RTKHIP_F32)
+    {
       c->pT16.Resize(npix);
+      c->maxBits.Resize(1);
+    }
     c->state.Resize(8);
     std::vector<double> mats, p2v, src;
     std::vector<int>    border;
@@ -903,7 +975,7 @@ rtkhip_cg_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const rtkhi
       const void * proj = q;
       if (stack_dtype != RTKHIP_F32)
       {
-        WeightAndStore(npix, q, dev_weights, c->pT16.get(), stack_dtype, s);
+        WeightAndStore(npix, q, dev_weights, c->pT16.get(), stack_dtype, c->maxBits.get(), s);
         proj = c->pT16.get();
       }
       else if (dev_weights)
@@ -916,6 +988,8 @@ rtkhip_cg_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const rtkhi
                                         stack_dtype, stream));
       else
         Throw(rtkhip_voxel_backproject(pd, vd, mats.data(), border.data(), nullptr, out, proj, stack_dtype, stream));
+      if (stack_dtype == RTKHIP_F16)
+        Unscale16(nvox, out, c->maxBits.get(), s);
     };
     // A x = BP(0, W FP(0, x))   (ReconstructionConjugateGradientOperator)
     auto applyA = [&](const float * x, float * out) {
