@@ -131,6 +131,34 @@ int rtkhip_joseph_backproject(const int proj_dim[3], const int vol_dim[3], const
 int rtkhip_fdk_weight(const int proj_dim[3], const double * geom7, const double proj_origin[2],
                       const double proj_spacing[2], const float * dev_in, float * dev_out, void * stream);
 
+/* Parker short-scan weighting.  Replaces CUDA_parker_weight
+ * (src/rtkCudaParkerShortScanImageFilter.cu:98-142) with the CPU filter's FP64
+ * weight math (include/rtkParkerShortScanImageFilter.hxx:162-188; weights rounded
+ * to FP32 like the reference's float weight image).
+ *   geom5          5 doubles per view: sdd, sourceOffsetX, projOffsetX, sid, gantry
+ *                  angle (rad), as src/rtkCudaParkerShortScanImageFilter.cxx:89-97 builds
+ *   delta, first_angle  the scan's Parker delta and first angle (rad)
+ *   proj_origin_u  physical u of buffer column 0; proj_spacing_u its pitch
+ *   in == out allowed. */
+int rtkhip_parker_weight(const int proj_dim[3], const double * geom5, double delta, double first_angle,
+                         double proj_origin_u, double proj_spacing_u, const float * dev_in, float * dev_out,
+                         void * stream);
+
+/* Displaced-detector weighting with zero padding.  Replaces CUDA_displaced_weight
+ * (src/rtkCudaDisplacedDetectorImageFilter.cu:142-185) with the CPU filter's FP64
+ * weights (include/rtkDisplacedDetectorImageFilter.hxx:250-290).
+ *   in_dim         {Nu, Nv, Nviews} of the input; the output is out_nu x Nv x Nviews
+ *   in_offset      output column of input column 0; columns outside the input are 0
+ *   geom4          4 doubles per view: sdd, sourceOffsetX, projOffsetX, sid
+ *                  (src/rtkCudaDisplacedDetectorImageFilter.cxx:112-119)
+ *   theta          min(-inferior corner, superior corner); positive_case: superior +
+ *                  inferior > 0 (weights rise toward +u)
+ *   proj_origin_u  physical u of input column 0
+ *   in == out allowed only without padding (out_nu == Nu, in_offset == 0). */
+int rtkhip_displaced_detector_weight(const int in_dim[3], int out_nu, int in_offset, const double * geom4,
+                                     double theta, int positive_case, double proj_origin_u, double proj_spacing_u,
+                                     const float * dev_in, float * dev_out, void * stream);
+
 /* Ramp filtering of every row of a projection slab: zero padding to `xpad`
  * (centred, optional Ohnesorge truncation mirror), R2C FFT, multiplication by
  * the kernel spectrum, C2R with 1/xpad, crop.  Replaces CUDA_padding +
@@ -269,6 +297,36 @@ int rtkhip_cg_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const r
  * draw: DrawConvexImageFilter (include/rtkDrawConvexImageFilter.hxx:41-61);
  * project: RayConvexIntersectionImageFilter chain (FP64 analytic chords,
  * include/rtkRayConvexIntersectionImageFilter.hxx:57-84). */
+/* rtk::ParkerShortScanImageFilter (include/rtkParkerShortScanImageFilter.hxx:41-209;
+ * CUDA: src/rtkCudaParkerShortScanImageFilter.cxx:28-116).  The scan is short when the
+ * largest gap between sorted gantry angles is >= threshold (rad; the reference's default
+ * is pi/9).  Not short: dev_out = dev_in (copied unless in == out).
+ *   is_short_scan  optional out
+ *   warn_view      optional out: first view whose half fan angle exceeds delta (the
+ *                  reference warns "not enough data"), -1 if none.  in == out allowed. */
+int rtkhip_parker_short_scan(const rtkhip_geometry * g, const rtkhip_image * stack, const float * dev_in,
+                             float * dev_out, double threshold, int * is_short_scan, int * warn_view, void * stream);
+
+/* Output region of rtk::DisplacedDetectorImageFilter
+ * (include/rtkDisplacedDetectorImageFilter.hxx:81-173): twice as wide, extended on the
+ * truncated side, when the detector is displaced and pad_on_truncated_side is set;
+ * otherwise the input's.  offsets_set: use min/max_offset instead of the geometry's
+ * projection offsets (SetOffsets).  corners (optional) receives the inferior and
+ * superior corners, weighting (optional) whether weights apply.  Errors as the
+ * reference: RTKHIP_ERR_UNSUPPORTED for cylindrical or parallel geometry and for
+ * displacements over 50% of the panel. */
+int rtkhip_displaced_detector_region(const rtkhip_geometry * g, const rtkhip_image * in, int disable,
+                                     int pad_on_truncated_side, int offsets_set, double min_offset,
+                                     double max_offset, rtkhip_image * out, double corners[2], int * weighting);
+
+/* rtk::DisplacedDetectorImageFilter (.hxx:175-323; CUDA:
+ * src/rtkCudaDisplacedDetectorImageFilter.cxx:28-137).  `out` must be the region
+ * rtkhip_displaced_detector_region returns for the same arguments.  Not displaced or
+ * disabled: dev_out = dev_in (copied unless in == out). */
+int rtkhip_displaced_detector(const rtkhip_geometry * g, const rtkhip_image * in, const float * dev_in,
+                              const rtkhip_image * out, float * dev_out, int disable, int pad_on_truncated_side,
+                              int offsets_set, double min_offset, double max_offset, void * stream);
+
 int rtkhip_draw_quadrics(const rtkhip_image * vol, const float * dev_in, float * dev_out, int nshapes,
                          const double * shapes, void * stream);
 int rtkhip_project_quadrics(rtkhip_context * c, const rtkhip_geometry * g, const rtkhip_image * stack,

@@ -268,6 +268,36 @@ struct Geometry
     return t;
   }
 
+  // .cxx:447-478: gap from each angle to the next one in sorted order (the last wraps).
+  std::vector<double> AngularGapsWithNext(const std::vector<double> & angles) const
+  {
+    const size_t        n = angles.size();
+    std::vector<double> g(n, 0.);
+    if (n == 1)
+      g[0] = 2 * kPi;
+    if (n < 2)
+      return g;
+    std::multimap<double, unsigned> s;
+    for (unsigned i = 0; i < n; i++)
+      s.insert({ angles[i], i });
+    auto curr = s.begin(), next = s.begin();
+    ++next;
+    for (; next != s.end(); ++curr, ++next)
+      g[curr->second] = next->first - curr->first;
+    g[curr->second] = s.begin()->first + 2 * kPi - curr->first;
+    return g;
+  }
+
+  // .cxx:641-663: coordinate on the untilted virtual detector through the isocentre.
+  double ToUntiltedCoordinateAtIsocenter(size_t i, double tilted) const
+  {
+    const double sidv = sid[i], sx = srcX[i];
+    const double sidu = std::sqrt(sidv * sidv + sx * sx);
+    const double l = (tilted + projX[i] - sx) * sidv / sdd[i] + sx;
+    const double cosa = sx / sidu;
+    return l * std::abs(sidv) / (sidu - l * cosa);
+  }
+
   // .cxx:481-533
   std::vector<double> AngularGaps(const std::vector<double> & angles) const
   {
@@ -627,6 +657,216 @@ FDKWeight(const Geometry & g, const ImageInfo & si, const T * in, T * out, int n
     {
       for (int i = 0; i < nu * nv; i++)
         po[i] = T(pi[i] * factor[k]);
+    }
+  });
+}
+
+// ------------------------------------------------------- Parker short scan --
+// include/rtkParkerShortScanImageFilter.hxx.  The stack's buffer column 0 sits at
+// physical u = si.origin[0]; slice kk is geometry entry firstProj + kk.
+struct ParkerInfo
+{
+  bool   isShortScan = false;
+  double delta = 0., firstAngle = 0.;
+  int    insufficientView = -1; // first view that triggers the reference's warning
+};
+
+// .hxx:41-110 (GenerateInputRequestedRegion)
+inline ParkerInfo
+ParkerSetup(const Geometry & g, const ImageInfo & si, double threshold)
+{
+  ParkerInfo                r;
+  const std::vector<double> gaps = g.AngularGapsWithNext(g.gantry);
+  const int                 n = int(gaps.size());
+  if (n == 0)
+    return r;
+  int maxPos = 0;
+  for (int i = 1; i < n; i++)
+    if (gaps[i] > gaps[maxPos])
+      maxPos = i;
+  if (g.sdd[0] == 0. || gaps[maxPos] < threshold)
+    return r;
+  r.isShortScan = true;
+  std::map<double, unsigned> sorted; // GetUniqueSortedAngles (.cxx:434-444)
+  for (unsigned i = 0; i < g.gantry.size(); i++)
+    sorted.insert({ g.gantry[i], i });
+  auto itLast = sorted.find(g.gantry[maxPos]);
+  auto itFirst = itLast;
+  itFirst = (++itFirst == sorted.end()) ? sorted.begin() : itFirst;
+  r.firstAngle = itFirst->first;
+  double lastAngle = itLast->first;
+  if (lastAngle < r.firstAngle)
+    lastAngle += 2 * kPi;
+  r.delta = 0.5 * (lastAngle - r.firstAngle - kPi);
+  r.delta = r.delta - 2 * kPi * std::floor(r.delta / (2 * kPi));
+  const double c1 = si.origin[0], c2 = si.origin[0] + si.spacing[0] * (si.size[0] - 1);
+  for (int k = 0; k < si.size[2]; k++)
+  {
+    const double sox = g.srcX[k], sid = g.sid[k];
+    const double invsid = 1. / std::sqrt(sid * sid + sox * sox);
+    const double hw = std::min(std::abs(g.ToUntiltedCoordinateAtIsocenter(k, c1)),
+                               std::abs(g.ToUntiltedCoordinateAtIsocenter(k, c2)));
+    if (r.delta < std::atan(hw * invsid))
+    {
+      r.insufficientView = k;
+      break;
+    }
+  }
+  return r;
+}
+
+// .hxx:112-209 (DynamicThreadedGenerateData): one row of weights per view in the
+// pixel type, multiplied into every row.  Not a short scan: out = in.
+template <class T>
+ParkerInfo
+ParkerShortScan(const Geometry & g, const ImageInfo & si, const T * in, T * out, double threshold, int nthreads,
+                int firstProj = 0)
+{
+  const ParkerInfo r = ParkerSetup(g, si, threshold);
+  const int        nu = si.size[0], nv = si.size[1];
+  const size_t     perView = size_t(nu) * nv;
+  if (!r.isShortScan)
+  {
+    if (in != out)
+      std::copy(in, in + perView * si.size[2], out);
+    return r;
+  }
+  ParallelFor(si.size[2], nthreads, [&](int kk) {
+    const int    k = kk + firstProj;
+    const double sox = g.srcX[k], sid = g.sid[k];
+    const double invsid = 1. / std::sqrt(sid * sid + sox * sox);
+    double       beta = g.gantry[k] - r.firstAngle;
+    if (beta < 0)
+      beta += 2 * kPi;
+    std::vector<T> w(nu);
+    double         point = si.origin[0];
+    for (int i = 0; i < nu; i++, point += si.spacing[0])
+    {
+      const double l = g.ToUntiltedCoordinateAtIsocenter(k, point);
+      const double alpha = std::atan(-1 * l * invsid);
+      const double d = r.delta;
+      if (beta <= 2 * d - 2 * alpha)
+        w[i] = T(2. * std::pow(std::sin((kPi * beta) / (4 * (d - alpha))), 2.));
+      else if (beta <= kPi - 2 * alpha)
+        w[i] = T(2.);
+      else if (beta <= kPi + 2 * d)
+        w[i] = T(2. * std::pow(std::sin((kPi * (kPi + 2 * d - beta)) / (4 * (d + alpha))), 2.));
+      else
+        w[i] = T(0.);
+    }
+    const T * pi = in + kk * perView;
+    T *       po = out + kk * perView;
+    for (int j = 0; j < nv; j++)
+      for (int i = 0; i < nu; i++)
+        po[size_t(j) * nu + i] = pi[size_t(j) * nu + i] * w[i];
+  });
+  return r;
+}
+
+// -------------------------------------------------------- displaced detector --
+// include/rtkDisplacedDetectorImageFilter.hxx.  The padded output is described by
+// its own ImageInfo (origin of its buffer column 0); input column i lands in
+// output column i + offset.
+struct DisplacedInfo
+{
+  ImageInfo out;
+  int       offset = 0;
+  double    inferior = 0., superior = 0.;
+  bool      weighting = false; // displaced and not disabled
+};
+
+// .hxx:81-173 (GenerateOutputInformation).  Throws where the reference throws.
+inline DisplacedInfo
+DisplacedDetectorSetup(const Geometry & g, const ImageInfo & si, bool disable, bool padOnTruncatedSide,
+                       bool offsetsSet, double minOffset, double maxOffset, double radius = 0.)
+{
+  DisplacedInfo r;
+  r.out = si;
+  if (disable)
+    return r;
+  if (radius != 0.)
+    throw std::runtime_error("Displaced detector cannot handle cylindrical detector.");
+  if (g.sdd.front() == 0.)
+    throw std::runtime_error("Displaced detector cannot handle parallel geometry.");
+  double inf = si.origin[0], sup = inf;
+  if (si.spacing[0] < 0.)
+    inf += si.spacing[0] * (si.size[0] - 1);
+  else
+    sup += si.spacing[0] * (si.size[0] - 1);
+  if (!offsetsSet)
+  {
+    double minSup = std::numeric_limits<double>::max();
+    double maxInf = -std::numeric_limits<double>::max();
+    for (size_t i = 0; i < g.size(); i++)
+    {
+      maxInf = std::max(maxInf, g.ToUntiltedCoordinateAtIsocenter(i, inf));
+      minSup = std::min(minSup, g.ToUntiltedCoordinateAtIsocenter(i, sup));
+    }
+    inf = maxInf;
+    sup = minSup;
+  }
+  else
+  {
+    inf += maxOffset;
+    sup += minOffset;
+  }
+  r.inferior = inf;
+  r.superior = sup;
+  if (inf > 0. || sup < 0.)
+    throw std::runtime_error("Cannot account for detector displacement larger than 50% of panel size.");
+  r.weighting = !(std::abs(inf + sup) < 0.1 * std::abs(sup - inf));
+  if (!r.weighting || !padOnTruncatedSide)
+    return r;
+  r.out.size[0] = 2 * si.size[0];
+  if (sup + inf > 0.)
+  {
+    r.offset = si.size[0];
+    r.out.origin[0] = si.origin[0] - si.spacing[0] * si.size[0];
+  }
+  return r;
+}
+
+// .hxx:175-323 (DynamicThreadedGenerateData): weights in double, zeros outside the
+// input's columns.  `out` has r.out's size.
+template <class T>
+void
+DisplacedDetector(const Geometry & g, const DisplacedInfo & r, const ImageInfo & si, const T * in, T * out,
+                  int nthreads, int firstProj = 0)
+{
+  const int    nu = si.size[0], nv = si.size[1], nuo = r.out.size[0];
+  const double theta = std::min(-1 * r.inferior, r.superior);
+  ParallelFor(si.size[2], nthreads, [&](int kk) {
+    const int           k = kk + firstProj;
+    std::vector<double> w(nu, 1.);
+    if (r.weighting)
+    {
+      const double sx = g.srcX[k];
+      double       sid = g.sid[k];
+      sid = std::sqrt(sid * sid + sx * sx);
+      double invsid = 0., invden = 0.;
+      if (sid != 0.)
+      {
+        invsid = 1. / sid;
+        invden = 1. / (2. * std::atan(theta * invsid));
+      }
+      double point = si.origin[0];
+      for (int i = 0; i < nu; i++, point += si.spacing[0])
+      {
+        const double l = g.ToUntiltedCoordinateAtIsocenter(k, point);
+        if (r.superior + r.inferior > 0.)
+          w[i] = (l <= -1 * theta) ? 0.0 : (l >= theta) ? 2.0 : std::sin(kPi * std::atan(l * invsid) * invden) + 1;
+        else
+          w[i] = (l <= -1 * theta) ? 2.0 : (l >= theta) ? 0.0 : 1 - std::sin(kPi * std::atan(l * invsid) * invden);
+      }
+    }
+    const T * pi = in + size_t(kk) * nu * nv;
+    T *       po = out + size_t(kk) * nuo * nv;
+    for (int j = 0; j < nv; j++)
+    {
+      T * row = po + size_t(j) * nuo;
+      std::fill(row, row + nuo, T(0));
+      for (int i = 0; i < nu; i++)
+        row[i + r.offset] = r.weighting ? T(pi[size_t(j) * nu + i] * w[i]) : pi[size_t(j) * nu + i];
     }
   });
 }

@@ -111,6 +111,52 @@ def fdk_weight(g9, stack, proj, first=0, threads=8):
     return out
 
 
+def untilted(g9, view, tilted):
+    """ThreeDCircularProjectionGeometry::ToUntiltedCoordinateAtIsocenter."""
+    n, g = _geom(g9)
+    out = ctypes.c_double()
+    _check(lib().rtkref_untilted(n, _ptr(g), int(view), ctypes.c_double(tilted), ctypes.byref(out)))
+    return out.value
+
+
+def parker(g9, stack, proj, threshold=np.pi / 9, threads=8):
+    """ParkerShortScanImageFilter: (weighted stack, is_short_scan, delta, first_angle,
+    first view lacking data or -1)."""
+    n, g = _geom(g9)
+    proj = np.ascontiguousarray(proj)
+    out = np.empty_like(proj)
+    s = ctypes.c_int(); df = np.zeros(2); w = ctypes.c_int()
+    fn = getattr(lib(), "rtkref_parker_" + _suffix(proj.dtype))
+    _check(fn(n, _ptr(g), ctypes.byref(stack), _ptr(proj), _ptr(out), ctypes.c_double(threshold), threads,
+              ctypes.byref(s), _ptr(df), ctypes.byref(w)))
+    return out, bool(s.value), df[0], df[1], w.value
+
+
+def displaced_setup(g9, stack, disable=False, pad=True, offsets=None):
+    """DisplacedDetectorImageFilter output information: (out image, column offset of the
+    input in it, (inferior, superior) corners, weighting applies)."""
+    n, g = _geom(g9)
+    out = Image(); off = ctypes.c_int(); c = np.zeros(2); wt = ctypes.c_int()
+    mn, mx = offsets if offsets is not None else (0., 0.)
+    _check(lib().rtkref_displaced_setup(n, _ptr(g), ctypes.byref(stack), int(disable), int(pad),
+                                        int(offsets is not None), ctypes.c_double(mn), ctypes.c_double(mx),
+                                        ctypes.byref(out), ctypes.byref(off), _ptr(c), ctypes.byref(wt)))
+    return out, off.value, (c[0], c[1]), bool(wt.value)
+
+
+def displaced(g9, stack, proj, disable=False, pad=True, offsets=None, threads=8):
+    """DisplacedDetectorImageFilter: (weighted, possibly zero-padded stack, out image)."""
+    n, g = _geom(g9)
+    out_im = displaced_setup(g9, stack, disable, pad, offsets)[0]
+    proj = np.ascontiguousarray(proj)
+    out = np.empty((proj.shape[0], proj.shape[1], out_im.size[0]), dtype=proj.dtype)
+    mn, mx = offsets if offsets is not None else (0., 0.)
+    fn = getattr(lib(), "rtkref_displaced_" + _suffix(proj.dtype))
+    _check(fn(n, _ptr(g), ctypes.byref(stack), _ptr(proj), _ptr(out), int(disable), int(pad),
+              int(offsets is not None), ctypes.c_double(mn), ctypes.c_double(mx), threads))
+    return out, out_im
+
+
 def ramp(stack, proj, threads=8, **ramp):
     proj = np.ascontiguousarray(proj)
     out = np.empty_like(proj)

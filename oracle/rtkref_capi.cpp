@@ -169,11 +169,62 @@ extern "C"
     });
   }
 
+  // Untilted coordinate at the isocentre of a physical detector coordinate.
+  int rtkref_untilted(int n, const double * g9, int view, double tilted, double * out)
+  {
+    return Guard([&] { *out = MakeGeometry(n, g9).ToUntiltedCoordinateAtIsocenter(view, tilted); });
+  }
+
+  // Output descriptor of the displaced-detector filter: out image, offset of the
+  // input's column 0 in it, corners {inferior, superior}, weighting flag.
+  int rtkref_displaced_setup(int n, const double * g9, const rtkref_image * stack, int disable, int pad,
+                             int offsetsSet, double minOffset, double maxOffset, rtkref_image * out, int * offset,
+                             double * corners, int * weighting)
+  {
+    return Guard([&] {
+      Geometry      g = MakeGeometry(n, g9);
+      DisplacedInfo r = DisplacedDetectorSetup(g, Info(stack), disable != 0, pad != 0, offsetsSet != 0, minOffset,
+                                               maxOffset);
+      for (int i = 0; i < 3; i++)
+      {
+        out->size[i] = r.out.size[i];
+        out->origin[i] = r.out.origin[i];
+        out->spacing[i] = r.out.spacing[i];
+      }
+      *offset = r.offset;
+      corners[0] = r.inferior;
+      corners[1] = r.superior;
+      *weighting = r.weighting;
+    });
+  }
+
 #define RTKREF_TYPED(T, S)                                                                                          \
   int rtkref_fdk_weight_##S(int n, const double * g9, const rtkref_image * stack, int first, const T * in, T * out, \
                             int threads)                                                                            \
   {                                                                                                                 \
     return Guard([&] { FDKWeight<T>(MakeGeometry(n, g9), Info(stack), in, out, threads, first); });                 \
+  }                                                                                                                 \
+  int rtkref_parker_##S(int n, const double * g9, const rtkref_image * stack, const T * in, T * out,              \
+                        double threshold, int threads, int * isShort, double * deltaFirst, int * warnView)          \
+  {                                                                                                                 \
+    return Guard([&] {                                                                                              \
+      ParkerInfo r = ParkerShortScan<T>(MakeGeometry(n, g9), Info(stack), in, out, threshold, threads);             \
+      *isShort = r.isShortScan;                                                                                     \
+      deltaFirst[0] = r.delta;                                                                                      \
+      deltaFirst[1] = r.firstAngle;                                                                                 \
+      *warnView = r.insufficientView;                                                                               \
+    });                                                                                                             \
+  }                                                                                                                 \
+  int rtkref_displaced_##S(int n, const double * g9, const rtkref_image * stack, const T * in, T * out,           \
+                           int disable, int pad, int offsetsSet, double minOffset, double maxOffset, int threads)   \
+  {                                                                                                                 \
+    return Guard([&] {                                                                                              \
+      Geometry      g = MakeGeometry(n, g9);                                                                        \
+      ImageInfo     si = Info(stack);                                                                               \
+      DisplacedInfo r = DisplacedDetectorSetup(g, si, disable != 0, pad != 0, offsetsSet != 0, minOffset,           \
+                                               maxOffset);                                                          \
+      DisplacedDetector<T>(g, r, si, in, out, threads);                                                             \
+    });                                                                                                             \
   }                                                                                                                 \
   int rtkref_ramp_##S(const rtkref_image * stack, const double * opts, const T * in, T * out, int threads)         \
   {                                                                                                                 \

@@ -11,10 +11,12 @@ from .filters import (  # noqa: F401
     ConstantImageSource,
     HipBackProjectionImageFilter,
     HipConjugateGradientConeBeamReconstructionFilter,
+    HipDisplacedDetectorImageFilter,
     HipFDKBackProjectionImageFilter,
     HipFDKConeBeamReconstructionFilter,
     HipForwardProjectionImageFilter,
     HipJosephBackProjectionImageFilter,
+    HipParkerShortScanImageFilter,
     Image,
     RampFilterOptions,
     ThreeDCircularProjectionGeometry,

@@ -79,6 +79,12 @@ SIGNATURES = [
     ("rtkhip_joseph_back_project", I, [P, P, ctypes.POINTER(Image), P, P, ctypes.POINTER(Image), P, I, P]),
     ("rtkhip_cg_reconstruct", I, [P, P, ctypes.POINTER(Image), P, ctypes.POINTER(Image), P, P, I, D, I, I, PD,
                                   PI3, P]),
+    ("rtkhip_parker_weight", I, [PI3, PD, D, D, D, D, P, P, P]),
+    ("rtkhip_displaced_detector_weight", I, [PI3, I, I, PD, D, I, D, D, P, P, P]),
+    ("rtkhip_parker_short_scan", I, [P, ctypes.POINTER(Image), P, P, D, PI3, PI3, P]),
+    ("rtkhip_displaced_detector_region", I, [P, ctypes.POINTER(Image), I, I, I, D, D, ctypes.POINTER(Image), PD,
+                                             PI3]),
+    ("rtkhip_displaced_detector", I, [P, ctypes.POINTER(Image), P, ctypes.POINTER(Image), P, I, I, I, D, D, P]),
     ("rtkhip_draw_quadrics", I, [ctypes.POINTER(Image), P, P, I, PD, P]),
     ("rtkhip_project_quadrics", I, [P, P, ctypes.POINTER(Image), P, P, I, PD, P]),
     ("rtkhip_cnn_num_params", I, []),

@@ -8,11 +8,11 @@ the output volume (applications/rtk3Doutputimage_section.ggo) and the projectors
 Geometry is RTK's XML, images are MetaImage.  Options that select processing outside
 this path (scanner raw-data corrections, cropping/binning, motion compensation,
 regularisation, masks) are accepted with their defaults and rejected with a message
-when set.  As in the reference application, the displaced-detector and Parker
-short-scan weightings run before FDK; they are identities for a full scan with a
-centred detector, which is what this build reconstructs: any other acquisition is
-rejected (see check_full_scan_centred), or run without that weighting when
---nodisplaced is given, as the reference does.
+when set.  As in the reference application with --hardware cuda
+(applications/rtkfdk/rtkfdk.cxx:83-113), rtkfdk runs the displaced-detector
+weighting (DisplacedDetectorImageFilter, the CUDA filter's semantics) and the Parker
+short-scan weighting before FDK; rtkconjugategradient weights the projections with
+the displaced-detector filter without padding, unless --nodisplaced is given.
 """
 import argparse
 import math
@@ -24,7 +24,8 @@ import torch
 
 from . import io
 from .filters import (ConstantImageSource, HipConjugateGradientConeBeamReconstructionFilter,
-                      HipFDKConeBeamReconstructionFilter, Image)
+                      HipDisplacedDetectorImageFilter, HipFDKConeBeamReconstructionFilter,
+                      HipParkerShortScanImageFilter, Image)
 
 
 def _add_input_projections(p):
@@ -110,50 +111,6 @@ def output_volume(args):
     return ConstantImageSource(tuple(size), tuple(spacing), tuple(origin))
 
 
-def check_full_scan_centred(geometry, stack, short_deg=20., nodisplaced=False):
-    """The decisions of rtk::ParkerShortScanImageFilter (max gap between consecutive
-    gantry angles below the short-scan threshold: no weighting,
-    include/rtkParkerShortScanImageFilter.hxx:46-55) and of
-    rtk::DisplacedDetectorImageFilter (untilted detector corners at the isocentre
-    symmetric within 10%: no weighting, include/rtkDisplacedDetectorImageFilter.hxx:
-    120-160, ToUntiltedCoordinateAtIsocenter at src/rtkThreeDCircularProjectionGeometry.cxx:
-    641-663).  Raises if either weighting would change the data."""
-    rows = geometry.rows()
-    if rows.size == 0:
-        raise SystemExit("empty geometry")
-    if np.any(rows[:, 1] == 0.):
-        raise SystemExit("parallel geometry is not supported by this build")
-    ang = np.sort(np.mod(np.radians(rows[:, 2]), 2 * math.pi))
-    gaps = np.diff(np.concatenate([ang, [ang[0] + 2 * math.pi]]))
-    if gaps.max() >= math.radians(short_deg):
-        raise SystemExit(f"short scan (largest angular gap {math.degrees(gaps.max()):.1f} deg >= --short "
-                         f"{short_deg}): Parker weighting is not implemented in this build")
-    if nodisplaced:
-        return
-    if geometry.GetRadiusCylindricalDetector() != 0.:
-        raise SystemExit("Displaced detector cannot handle cylindrical detector; use --nodisplaced")
-    nu = stack.tensor.shape[2]
-    inf = stack.origin[0]
-    sup = inf + stack.spacing[0] * (nu - 1)
-    if stack.spacing[0] < 0:
-        inf, sup = sup, inf
-
-    def untilt(r, c):
-        sid, sdd, sx, px = r[0], r[1], r[7], r[3]
-        sidu = math.sqrt(sid * sid + sx * sx)
-        l = (c + px - sx) * sid / sdd + sx
-        return l * abs(sid) / (sidu - l * sx / sidu)
-
-    inf = max(untilt(r, inf) for r in rows)
-    sup = min(untilt(r, sup) for r in rows)
-    if inf > 0. or sup < 0.:
-        raise SystemExit(f"Cannot account for detector displacement larger than 50% of panel size "
-                         f"(corner inf={inf}, sup={sup})")
-    if abs(inf + sup) >= 0.1 * abs(sup - inf):
-        raise SystemExit("displaced detector: its weighting is not implemented in this build; "
-                         "pass --nodisplaced to reconstruct without it")
-
-
 def _write(args, img, verbose):
     t = img.tensor.float().cpu().numpy()
     io.write_image(args.output, t, img.origin, img.spacing)
@@ -186,10 +143,21 @@ def rtkfdk(argv=None):
         raise SystemExit("--signal/--dvf: motion compensation is not part of this build")
     proj = read_projections(args, args.verbose)
     g = io.read_geometry(args.geometry)
-    check_full_scan_centred(g, proj, args.short, args.nodisplaced)
+    ddf = HipDisplacedDetectorImageFilter()
+    ddf.SetInput(proj)
+    ddf.SetGeometry(g)
+    ddf.SetDisable(args.nodisplaced)
+    ddf.InPlaceOn()
+    ddf.Update()
+    pssf = HipParkerShortScanImageFilter()
+    pssf.SetInput(ddf.GetOutput())
+    pssf.SetGeometry(g)
+    pssf.SetAngularGapThreshold(args.short * math.pi / 180.)
+    pssf.InPlaceOn()  # the stack read from disk is not used again
+    pssf.Update()
     f = HipFDKConeBeamReconstructionFilter()
     f.SetInput(0, output_volume(args))
-    f.SetInput(1, proj)
+    f.SetInput(1, pssf.GetOutput())
     f.SetGeometry(g)
     f.SetProjectionSubsetSize(args.subsetsize)
     f.GetRampFilter().SetTruncationCorrection(args.pad)
@@ -242,7 +210,6 @@ def rtkconjugategradient(argv=None):
         raise SystemExit("--regweights/--mask/--gamma/--tikhonov: regularised CG is not part of this build")
     proj = read_projections(args, args.verbose)
     g = io.read_geometry(args.geometry)
-    check_full_scan_centred(g, proj, 180., args.nodisplaced)
     if args.input:
         a, o, s = io.read_image(args.input)
         x0 = Image(torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda(), o, s)
@@ -256,6 +223,7 @@ def rtkconjugategradient(argv=None):
     f.SetNumberOfIterations(args.niterations)
     f.SetResidualTolerance(args.tolerance)
     f.SetStackDtype(args.stackdtype)
+    f.SetDisableDisplacedDetectorFilter(args.nodisplaced)
     if args.weights:
         a, o, s = io.read_image(args.weights)
         if a.ndim == 2:

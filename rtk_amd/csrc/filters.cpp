@@ -24,6 +24,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <string>
 #include <type_traits>
@@ -242,6 +243,35 @@ struct rtkhip_geometry
         rt[4 * r + c] = v.rot[4 * c + r];
     rt[15] = 1.;
     return rtkhip::Mul(rt, d);
+  }
+
+  // ToUntiltedCoordinateAtIsocenter (.cxx:641-663).
+  double Untilted(size_t i, double tilted) const
+  {
+    const View & v = views[i];
+    const double sidu = std::sqrt(v.sid * v.sid + v.srcX * v.srcX);
+    const double l = (tilted + v.projX - v.srcX) * v.sid / v.sdd + v.srcX;
+    const double cosa = v.srcX / sidu;
+    return l * std::abs(v.sid) / (sidu - l * cosa);
+  }
+
+  // GetAngularGapsWithNext over the gantry angles (.cxx:447-478).
+  std::vector<double> GantryGapsWithNext() const
+  {
+    const size_t        n = views.size();
+    std::vector<double> g(n, 0.);
+    if (n == 1)
+      g[0] = 2 * kPi;
+    if (n < 2)
+      return g;
+    std::vector<std::pair<double, unsigned>> s(n);
+    for (unsigned i = 0; i < n; i++)
+      s[i] = { views[i].gantry, i };
+    std::stable_sort(s.begin(), s.end(), [](const auto & a, const auto & b) { return a.first < b.first; });
+    for (size_t k = 0; k + 1 < n; k++)
+      g[s[k].second] = s[k + 1].first - s[k].first;
+    g[s[n - 1].second] = s[0].first + 2 * kPi - s[n - 1].first;
+    return g;
   }
 
   // GetAngularGaps over the source angles (.cxx:481-533).
@@ -1069,5 +1099,235 @@ rtkhip_project_quadrics(rtkhip_context * c, const rtkhip_geometry * g, const rtk
     if (np > 0)
       ProjectQuadricsLaunch(stack->size[0], stack->size[1], np, rays.data(), nshapes, shapes, dev_in, dev_out,
                             Stream(stream));
+  });
+}
+
+// ------------------------------------------------- short-scan / displaced detector --
+namespace rtkhip
+{
+namespace
+{
+
+struct ParkerSetupResult
+{
+  bool   isShortScan = false;
+  double delta = 0., firstAngle = 0.;
+  int    warnView = -1;
+};
+
+// include/rtkParkerShortScanImageFilter.hxx:41-110.  Corners are those of the
+// stack's buffered region (= its largest possible region here).
+ParkerSetupResult
+ParkerSetup(const rtkhip_geometry & g, const rtkhip_image & stack, double threshold)
+{
+  ParkerSetupResult         r;
+  const std::vector<double> gaps = g.GantryGapsWithNext();
+  const int                 n = int(gaps.size());
+  if (n == 0)
+    return r;
+  int maxPos = 0;
+  for (int i = 1; i < n; i++)
+    if (gaps[i] > gaps[maxPos])
+      maxPos = i;
+  if (g.views[0].sdd == 0. || gaps[maxPos] < threshold)
+    return r;
+  r.isShortScan = true;
+  // GetUniqueSortedAngles (.cxx:434-444): the angle after the largest gap, cyclically
+  std::vector<double> sorted(n);
+  for (int i = 0; i < n; i++)
+    sorted[i] = g.views[i].gantry;
+  std::sort(sorted.begin(), sorted.end());
+  sorted.erase(std::unique(sorted.begin(), sorted.end()), sorted.end());
+  const double lastA = g.views[maxPos].gantry;
+  auto         it = std::upper_bound(sorted.begin(), sorted.end(), lastA);
+  r.firstAngle = (it == sorted.end()) ? sorted.front() : *it;
+  double last = lastA;
+  if (last < r.firstAngle)
+    last += 2 * kPi;
+  r.delta = 0.5 * (last - r.firstAngle - kPi);
+  r.delta = r.delta - 2 * kPi * std::floor(r.delta / (2 * kPi));
+  const double c1 = stack.origin[0] + stack.spacing[0] * stack.index[0];
+  const double c2 = c1 + stack.spacing[0] * (stack.size[0] - 1);
+  for (int k = 0; k < stack.size[2]; k++)
+  {
+    const auto & v = g.views[k];
+    const double invsid = 1. / std::sqrt(v.sid * v.sid + v.srcX * v.srcX);
+    const double hw = std::min(std::abs(g.Untilted(k, c1)), std::abs(g.Untilted(k, c2)));
+    if (r.delta < std::atan(hw * invsid))
+    {
+      r.warnView = k;
+      break;
+    }
+  }
+  return r;
+}
+
+struct DisplacedSetupResult
+{
+  rtkhip_image out;
+  int          offset = 0; // output column of the input's column 0
+  double       inferior = 0., superior = 0.;
+  bool         weighting = false;
+};
+
+// include/rtkDisplacedDetectorImageFilter.hxx:81-173 (GenerateOutputInformation).
+DisplacedSetupResult
+DisplacedSetup(const rtkhip_geometry & g, const rtkhip_image & in, bool disable, bool pad, bool offsetsSet,
+               double minOffset, double maxOffset)
+{
+  DisplacedSetupResult r;
+  r.out = in;
+  if (disable)
+    return r;
+  RTKHIP_REQUIRE(g.radius == 0., RTKHIP_ERR_UNSUPPORTED,
+                 "Displaced detector cannot handle cylindrical detector. Consider disabling it by setting "
+                 "m_Disable=true or using the nodisplaced flag of the application you are running");
+  RTKHIP_REQUIRE(!g.views.empty() && g.views.front().sdd != 0., RTKHIP_ERR_UNSUPPORTED,
+                 "Displaced detector cannot handle parallel geometry. Consider disabling it by setting "
+                 "m_Disable=true or using the nodisplaced flag of the application you are running");
+  double inf = in.origin[0] + in.spacing[0] * in.index[0], sup = inf;
+  if (in.spacing[0] < 0.)
+    inf += in.spacing[0] * (in.size[0] - 1);
+  else
+    sup += in.spacing[0] * (in.size[0] - 1);
+  if (!offsetsSet)
+  {
+    double minSup = std::numeric_limits<double>::max(), maxInf = -std::numeric_limits<double>::max();
+    for (size_t i = 0; i < g.views.size(); i++)
+    {
+      maxInf = std::max(maxInf, g.Untilted(i, inf));
+      minSup = std::min(minSup, g.Untilted(i, sup));
+    }
+    inf = maxInf;
+    sup = minSup;
+  }
+  else
+  {
+    inf += maxOffset;
+    sup += minOffset;
+  }
+  r.inferior = inf;
+  r.superior = sup;
+  RTKHIP_REQUIRE(!(inf > 0. || sup < 0.), RTKHIP_ERR_UNSUPPORTED,
+                 "Cannot account for detector displacement larger than 50% of panel size. Corner inf=" +
+                   std::to_string(inf) + " and corner sup=" + std::to_string(sup));
+  r.weighting = !(std::abs(inf + sup) < 0.1 * std::abs(sup - inf));
+  if (!r.weighting || !pad)
+    return r;
+  r.out.size[0] = 2 * in.size[0];
+  if (sup + inf > 0.)
+  {
+    r.out.index[0] = in.index[0] - in.size[0];
+    r.offset = in.size[0];
+  }
+  return r;
+}
+
+// Copies a stack of n floats unless in == out.
+void
+CopyStack(const float * in, float * out, size_t n, hipStream_t s)
+{
+  if (in != out && n)
+    RTKHIP_CHECK(hipMemcpyAsync(out, in, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+}
+
+} // namespace
+} // namespace rtkhip
+
+extern "C" int
+rtkhip_parker_short_scan(const rtkhip_geometry * g, const rtkhip_image * stack, const float * dev_in, float * dev_out,
+                         double threshold, int * is_short_scan, int * warn_view, void * stream)
+{
+  using namespace rtkhip;
+  return Guard([&] {
+    RTKHIP_REQUIRE(dev_in && dev_out, RTKHIP_ERR_INVALID_ARGUMENT, "parker_short_scan: null buffer");
+    CheckImage(stack, "parker_short_scan projections");
+    CheckGeometry(g, *stack, false);
+    RequireDevice();
+    const ParkerSetupResult r = ParkerSetup(*g, *stack, threshold);
+    if (is_short_scan)
+      *is_short_scan = r.isShortScan;
+    if (warn_view)
+      *warn_view = r.warnView;
+    hipStream_t  s = Stream(stream);
+    const size_t n = size_t(stack->size[0]) * stack->size[1] * stack->size[2];
+    if (!r.isShortScan)
+      CopyStack(dev_in, dev_out, n, s);
+    else
+    {
+      std::vector<double> g5(5 * size_t(stack->size[2]));
+      for (int p = 0; p < stack->size[2]; p++)
+      {
+        const auto & v = g->views[size_t(stack->index[2]) + p];
+        const double row[5] = { v.sdd, v.srcX, v.projX, v.sid, v.gantry };
+        std::copy(row, row + 5, g5.begin() + 5 * size_t(p));
+      }
+      const int rc = rtkhip_parker_weight(stack->size, g5.data(), r.delta, r.firstAngle,
+                                          stack->origin[0] + stack->spacing[0] * stack->index[0], stack->spacing[0],
+                                          dev_in, dev_out, stream);
+      RTKHIP_REQUIRE(rc == RTKHIP_OK, rc, rtkhip_last_error());
+    }
+    RTKHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+extern "C" int
+rtkhip_displaced_detector_region(const rtkhip_geometry * g, const rtkhip_image * in, int disable,
+                                 int pad_on_truncated_side, int offsets_set, double min_offset, double max_offset,
+                                 rtkhip_image * out, double corners[2], int * weighting)
+{
+  using namespace rtkhip;
+  return Guard([&] {
+    RTKHIP_REQUIRE(out, RTKHIP_ERR_INVALID_ARGUMENT, "displaced_detector_region: null output descriptor");
+    CheckImage(in, "displaced_detector projections");
+    RTKHIP_REQUIRE(g, RTKHIP_ERR_GEOMETRY, "Geometry has not been set.");
+    const DisplacedSetupResult r =
+      DisplacedSetup(*g, *in, disable != 0, pad_on_truncated_side != 0, offsets_set != 0, min_offset, max_offset);
+    *out = r.out;
+    if (corners)
+    {
+      corners[0] = r.inferior;
+      corners[1] = r.superior;
+    }
+    if (weighting)
+      *weighting = r.weighting;
+  });
+}
+
+extern "C" int
+rtkhip_displaced_detector(const rtkhip_geometry * g, const rtkhip_image * in, const float * dev_in,
+                          const rtkhip_image * out, float * dev_out, int disable, int pad_on_truncated_side,
+                          int offsets_set, double min_offset, double max_offset, void * stream)
+{
+  using namespace rtkhip;
+  return Guard([&] {
+    RTKHIP_REQUIRE(dev_in && dev_out && out, RTKHIP_ERR_INVALID_ARGUMENT, "displaced_detector: null argument");
+    CheckImage(in, "displaced_detector projections");
+    CheckGeometry(g, *in, false);
+    RequireDevice();
+    const DisplacedSetupResult r =
+      DisplacedSetup(*g, *in, disable != 0, pad_on_truncated_side != 0, offsets_set != 0, min_offset, max_offset);
+    for (int i = 0; i < 3; i++)
+      RTKHIP_REQUIRE(out->size[i] == r.out.size[i] && out->index[i] == r.out.index[i], RTKHIP_ERR_INVALID_ARGUMENT,
+                     "displaced_detector: output region differs from rtkhip_displaced_detector_region's");
+    hipStream_t s = Stream(stream);
+    if (!r.weighting)
+      CopyStack(dev_in, dev_out, size_t(in->size[0]) * in->size[1] * in->size[2], s);
+    else
+    {
+      const double theta = std::min(-1 * r.inferior, r.superior);
+      std::vector<double> g4(4 * size_t(in->size[2]));
+      for (int p = 0; p < in->size[2]; p++)
+      {
+        const auto & v = g->views[size_t(in->index[2]) + p];
+        const double row[4] = { v.sdd, v.srcX, v.projX, v.sid };
+        std::copy(row, row + 4, g4.begin() + 4 * size_t(p));
+      }
+      const int rc = rtkhip_displaced_detector_weight(
+        in->size, r.out.size[0], r.offset, g4.data(), theta, (r.superior + r.inferior > 0.) ? 1 : 0,
+        in->origin[0] + in->spacing[0] * in->index[0], in->spacing[0], dev_in, dev_out, stream);
+      RTKHIP_REQUIRE(rc == RTKHIP_OK, rc, rtkhip_last_error());
+    }
+    RTKHIP_CHECK(hipStreamSynchronize(s));
   });
 }

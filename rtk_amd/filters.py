@@ -356,6 +356,112 @@ class HipFDKBackProjectionImageFilter(HipBackProjectionImageFilter):
     _fdk = 1
 
 
+class HipParkerShortScanImageFilter(_Filter):
+    """rtk::ParkerShortScanImageFilter / CudaParkerShortScanImageFilter: weights a short
+    scan's projections (include/rtkParkerShortScanImageFilter.hxx:41-209).  A scan is
+    short when the largest gap between sorted gantry angles reaches the threshold."""
+
+    def __init__(self):
+        super().__init__()
+        self._threshold = np.pi / 9  # reference default (rtkParkerShortScanImageFilter.h:113)
+        self._short = False
+        self._warn = -1
+
+    def SetAngularGapThreshold(self, t):
+        self._threshold = float(t)
+
+    def GetAngularGapThreshold(self):
+        return self._threshold
+
+    def IsShortScan(self):
+        return self._short
+
+    def Update(self):
+        import warnings
+        proj = self._inputs[0]
+        _require_cuda(proj, "Parker projections")
+        if proj.tensor.dtype != torch.float32:
+            raise ValueError("Parker weighting expects an FP32 stack")
+        out = self._out_for(proj)
+        pd = proj.descriptor()
+        s, w = ctypes.c_int(), ctypes.c_int()
+        check(lib().rtkhip_parker_short_scan(self._geom(), ctypes.byref(pd), _ptr(proj.tensor), _ptr(out),
+                                             self._threshold, ctypes.byref(s), ctypes.byref(w), _stream()))
+        self._short, self._warn = bool(s.value), w.value
+        if self._warn >= 0:
+            warnings.warn("You do not have enough data for proper Parker weighting (short scan) according to "
+                          f"projection #{self._warn}")
+        self._output = proj.like(out)
+
+
+class HipDisplacedDetectorImageFilter(_Filter):
+    """rtk::DisplacedDetectorImageFilter / CudaDisplacedDetectorImageFilter: weights
+    projections of a laterally displaced detector and, by default, zero-pads them on
+    the truncated side to twice their width
+    (include/rtkDisplacedDetectorImageFilter.hxx:81-323)."""
+
+    def __init__(self):
+        super().__init__()
+        self._disable = False
+        self._pad = True
+        self._offsets = None
+        self._corners = (0., 0.)
+        self._weighting = False
+
+    def SetDisable(self, v):
+        self._disable = bool(v)
+
+    def GetDisable(self):
+        return self._disable
+
+    def SetPadOnTruncatedSide(self, v):
+        self._pad = bool(v)
+
+    def GetPadOnTruncatedSide(self):
+        return self._pad
+
+    def SetOffsets(self, min_offset, max_offset):
+        self._offsets = (float(min_offset), float(max_offset))
+
+    def GetInferiorCorner(self):
+        return self._corners[0]
+
+    def GetSuperiorCorner(self):
+        return self._corners[1]
+
+    def _args(self):
+        mn, mx = self._offsets if self._offsets is not None else (0., 0.)
+        return int(self._disable), int(self._pad), int(self._offsets is not None), mn, mx
+
+    def OutputRegion(self, proj):
+        """Output descriptor (size, index) and whether weights apply (GenerateOutputInformation)."""
+        pd = proj.descriptor()
+        od = _lib.Image()
+        c = (ctypes.c_double * 2)()
+        wt = ctypes.c_int()
+        check(lib().rtkhip_displaced_detector_region(self._geom(), ctypes.byref(pd), *self._args(), ctypes.byref(od),
+                                                     c, ctypes.byref(wt)))
+        self._corners = (c[0], c[1])
+        self._weighting = bool(wt.value)
+        return od
+
+    def Update(self):
+        proj = self._inputs[0]
+        _require_cuda(proj, "displaced-detector projections")
+        if proj.tensor.dtype != torch.float32:
+            raise ValueError("displaced-detector weighting expects an FP32 stack")
+        od = self.OutputRegion(proj)
+        size = tuple(od.size)
+        if size == proj.size and self._inplace:
+            out = proj.tensor
+        else:
+            out = torch.empty((size[2], size[1], size[0]), dtype=torch.float32, device=proj.tensor.device)
+        pd = proj.descriptor()
+        check(lib().rtkhip_displaced_detector(self._geom(), ctypes.byref(pd), _ptr(proj.tensor), ctypes.byref(od),
+                                              _ptr(out), *self._args(), _stream()))
+        self._output = Image(out, proj.origin, proj.spacing, tuple(od.index))
+
+
 class HipConjugateGradientConeBeamReconstructionFilter(_Filter):
     """rtk::ConjugateGradientConeBeamReconstructionFilter with FP_JOSEPH and BP_VOXELBASED or BP_JOSEPH:
     input 0 = X0, input 1 = projections; optional weights
@@ -372,6 +478,15 @@ class HipConjugateGradientConeBeamReconstructionFilter(_Filter):
         self._resid = np.zeros(0)
         self._iters = 0
         self._dtype = "f32"
+        self._disable_dd = False
+
+    def SetDisableDisplacedDetectorFilter(self, v):
+        """The weights are displaced-detector weighted (without padding) unless disabled
+        (rtkConjugateGradientConeBeamReconstructionFilter.hxx:41,54,222-242)."""
+        self._disable_dd = bool(v)
+
+    def GetDisableDisplacedDetectorFilter(self):
+        return self._disable_dd
 
     def SetStackDtype(self, name):
         """Storage of the stacks the backprojector reads: "f32", "f16" or "bf16"
@@ -411,6 +526,16 @@ class HipConjugateGradientConeBeamReconstructionFilter(_Filter):
         it = ctypes.c_int()
         vd, pd = vol.descriptor(), proj.descriptor()
         w = self._weights.tensor if self._weights is not None else None
+        ddf = HipDisplacedDetectorImageFilter()
+        ddf.SetGeometry(self._geometry)
+        ddf.SetDisable(self._disable_dd)
+        ddf.SetPadOnTruncatedSide(False)
+        ddf.OutputRegion(proj)
+        if ddf._weighting:
+            ddf.SetInput(self._weights if self._weights is not None else
+                         proj.like(torch.ones_like(proj.tensor, dtype=torch.float32)))
+            ddf.Update()
+            w = ddf.GetOutput().tensor
         check(lib().rtkhip_cg_reconstruct(_context(), self._geom(), ctypes.byref(vd), _ptr(x), ctypes.byref(pd),
                                           _ptr(proj.tensor), _ptr(w), self._niter, self._tol, self._bp,
                                           _DTYPES[self._dtype][0], res.ctypes.data_as(_lib.PD), ctypes.byref(it),

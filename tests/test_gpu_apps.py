@@ -66,6 +66,45 @@ def test_rtkconjugategradient_cli(dataset, hip):
     assert rel_l2(vol, f.GetOutput().numpy()) <= 1e-5
 
 
+def test_rtkfdk_cli_short_scan_displaced_detector(tmp_path, oracle, hip):
+    """A 220 degree arc on a detector shifted by 40 mm: rtkfdk runs the displaced-
+    detector and Parker weightings before FDK (applications/rtkfdk/rtkfdk.cxx:83-113)
+    and writes what the filter chain gives; --nodisplaced skips the first one."""
+    from rtk_amd import apps, io, phantoms
+    n = 60
+    rows = np.array([[1000., 1536., i * 220. / n, 40., 0, 0, 0, 0, 0] for i in range(n)])
+    q = phantoms.shepp_logan(60.)
+    os_ = oracle.image((NU, NU, n), SORIG + (0.,), SSP + (1.,))
+    proj = oracle.project_quadrics(rows, os_, np.zeros((n, NU, NU)), q).astype(np.float32)
+    for i in range(n):
+        io.write_image(str(tmp_path / f"p{i:03d}.mha"), proj[i], SORIG, SSP)
+    io.write_geometry(hip_geometry(hip, rows), str(tmp_path / "g.xml"))
+    base = ["-g", str(tmp_path / "g.xml"), "-p", str(tmp_path), "-r", r"p\d+\.mha", "--size", "64", "--spacing", "2"]
+    for nodisplaced in (False, True):
+        out = str(tmp_path / f"fdk{int(nodisplaced)}.mha")
+        assert apps.rtkfdk(base + ["-o", out] + (["--nodisplaced"] if nodisplaced else [])) == 0
+        vol = io.read_image(out)[0]
+        g = hip_geometry(hip, rows)
+        ddf = hip.HipDisplacedDetectorImageFilter()
+        ddf.SetInput(to_device(hip, proj, SORIG + (0.,), SSP + (1.,)))
+        ddf.SetGeometry(g)
+        ddf.SetDisable(nodisplaced)
+        ddf.Update()
+        assert ddf.GetOutput().size[0] == (NU if nodisplaced else 2 * NU)
+        pssf = hip.HipParkerShortScanImageFilter()
+        pssf.SetInput(ddf.GetOutput())
+        pssf.SetGeometry(g)
+        pssf.Update()
+        assert pssf.IsShortScan()
+        f = hip.HipFDKConeBeamReconstructionFilter()
+        f.SetInput(0, hip.ConstantImageSource((64,) * 3, (2.,) * 3, (-63.,) * 3))
+        f.SetInput(1, pssf.GetOutput())
+        f.SetGeometry(g)
+        f.SetProjectionSubsetSize(16)
+        f.Update()
+        assert np.array_equal(vol, f.GetOutput().numpy())
+
+
 def test_cli_rejects_processing_outside_the_path(dataset, hip):
     from rtk_amd import apps
     d, _ = dataset

@@ -1,17 +1,14 @@
-"""RTK geometry XML and MetaImage I/O, and the applications' acquisition checks (CPU).
+"""RTK geometry XML and MetaImage I/O (CPU).
 
 The geometry writer/reader follow src/rtkThreeDCircularProjectionGeometryXMLFile
 {Writer,Reader}.cxx; the hand-written document below exercises the reader rules
 those files define (global parameters, values persisting across projections,
 case-insensitive names, the "Angle" alias, the matrix consistency check)."""
-import math
 import os
 import zlib
 
 import numpy as np
 import pytest
-
-from helpers import circular_rows
 
 
 def _geom(rows):
@@ -125,27 +122,3 @@ def test_projection_file_order(tmp_path):
     names = [os.path.basename(f) for f in io.projection_files(str(tmp_path), r"proj_(\d+)\.mha", nsort=True,
                                                             submatch=1)]
     assert names == ["proj_1.mha", "proj_9.mha", "proj_10.mha", "proj_100.mha"]
-
-
-class _Stack:
-    def __init__(self, nu, origin_u, spacing_u=0.8):
-        import torch
-        self.tensor = torch.zeros((1, 4, nu))
-        self.origin = (origin_u, 0., 0.)
-        self.spacing = (spacing_u, 0.8, 1.)
-
-
-def test_full_scan_and_centred_detector_checks():
-    from rtk_amd.apps import check_full_scan_centred
-    full = _geom(circular_rows(90, 1000., 1536.))
-    check_full_scan_centred(full, _Stack(256, -102.), 20.)
-    short = _geom(circular_rows(90, 1000., 1536., arc=200.))
-    with pytest.raises(SystemExit, match="short scan"):
-        check_full_scan_centred(short, _Stack(256, -102.), 20.)
-    # detector shifted by a quarter of its width: displaced-detector weighting would apply
-    with pytest.raises(SystemExit, match="displaced detector"):
-        check_full_scan_centred(full, _Stack(256, -102. + 60.), 20.)
-    check_full_scan_centred(full, _Stack(256, -102. + 60.), 20., nodisplaced=True)
-    with pytest.raises(SystemExit, match="50%"):
-        check_full_scan_centred(full, _Stack(256, 10.), 20.)
-    assert math.isclose(math.degrees(2 * math.pi / 90), 4.0)
