@@ -120,6 +120,55 @@ def projector_pair_measurement(rtk, np, torch, g, q, proj, vol, reps=2):
     return out
 
 
+def weighting_measurement(rtk, np, torch, proj, reps=3):
+    """Parker short-scan and displaced-detector weighting of the 720 x 1024^2 stack
+    (HBM streams): a 240 degree arc for Parker (stack in, stack out), a detector
+    shifted by 60 mm for the displaced-detector filter (stack in, twice as wide out).
+    Kernel time from HIP events on the launch stream."""
+    out = {"workload": "720 x 1024^2 FP32 stack: Parker weights (240 deg arc); displaced-detector weights "
+                       "(+60 mm detector shift, zero-padded to 2048 columns)"}
+    n = proj.tensor.shape[0]
+    for name, rows, cls in (
+            ("parker", [[1000., 1536., i * 240. / n, 0., 0., 0., 0., 0., 0.] for i in range(n)],
+             rtk.HipParkerShortScanImageFilter),
+            ("displaced_detector", [[1000., 1536., i * 360. / n, 60., 0., 0., 0., 0., 0.] for i in range(n)],
+             rtk.HipDisplacedDetectorImageFilter)):
+        g = rtk.ThreeDCircularProjectionGeometry()
+        for r in rows:
+            g.AddProjection(*r)
+        f = cls()
+        f.SetInput(proj)
+        f.SetGeometry(g)
+        f.Update()
+        nbytes = proj.tensor.numel() * 4 + f.GetOutput().tensor.numel() * 4
+        ms = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            f.Update()
+            e1.record()
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        t = min(ms)
+        out[name] = {"ms": round(t, 3), "bytes": int(nbytes), "gb_per_s": round(nbytes / t / 1e6, 1),
+                     "frac_of_8tb": round(nbytes / t / 1e6 / 8000., 3)}
+        del f
+    # a plain device copy of the stack for scale (what a pure stream reaches here)
+    dst = torch.empty_like(proj.tensor)
+    ms = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(proj.tensor)
+        e1.record()
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    out["device_copy_gb_per_s"] = round(2 * proj.tensor.numel() * 4 / min(ms) / 1e6, 1)
+    del dst
+    log(f"weighting: {out}")
+    return out
+
+
 def cg_measurement(rtk, np, torch, max_iter, tol):
     """Plain CG on 256^3 @1 mm from 360 views of 512^2 @0.8 mm with Poisson noise
     (I0 = 1e4): 30 iterations from zero with RTK's default pair (Joseph FP + voxel
@@ -215,6 +264,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true", help="skip the CG and warm-start measurements")
     ap.add_argument("--no-pair", action="store_true", help="skip the Joseph FP / voxel BP measurement")
+    ap.add_argument("--no-weights", action="store_true", help="skip the Parker / displaced-detector measurement")
     ap.add_argument("--cg-tol", type=float, default=1e-3)
     ap.add_argument("--cg-iter", type=int, default=100)
     ap.add_argument("--warm-seeds", type=int, default=2, help="held-out seeds for the warm-start measurement")
@@ -300,6 +350,8 @@ def main():
     }
     if not args.no_pair:
         result["projector_pair"] = projector_pair_measurement(rtk, np, torch, g, q, proj, vol)
+    if not args.no_weights:
+        result["short_scan_weighting"] = weighting_measurement(rtk, np, torch, proj)
     if not args.no_cg:
         result["cg"] = cg_measurement(rtk, np, torch, args.cg_iter, args.cg_tol)
         result["cg_warm_start"] = warm_start_measurement(rtk, np, torch, _warm_seeds(args), args.cg_tol)

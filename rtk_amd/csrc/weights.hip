@@ -99,6 +99,21 @@ ColumnWeight(const RowView & v, double u, double p0)
   }
 }
 
+// Streaming (non-temporal) 16-byte accesses: the stack is touched once.
+typedef float Vec4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ Vec4
+LoadStream(const float * p)
+{
+  return __builtin_nontemporal_load(reinterpret_cast<const Vec4 *>(p));
+}
+
+__device__ __forceinline__ void
+StoreStream(float * p, Vec4 v)
+{
+  __builtin_nontemporal_store(v, reinterpret_cast<Vec4 *>(p));
+}
+
 template <int MODE>
 __device__ __forceinline__ float
 Apply(float x, double w)
@@ -139,17 +154,30 @@ __global__ __launch_bounds__(256) void row_weight_kernel(const RowView * __restr
   const size_t inView = size_t(k) * nv * nuIn, outView = size_t(k) * nv * nuOut;
   if constexpr (VEC)
   {
-    const bool any = inside[0];
-    for (int r = rs; r < nv; r += rowSplit)
+    const bool    any = inside[0];
+    const float * src = in + inView + (j0 - offset);
+    float *       dst = out + outView + j0;
+    // Four rows per step: all loads issue before the stores (rows are distinct, so
+    // this is safe when in == out), keeping 64 B per lane in flight.
+    constexpr int U = 4;
+    int           r = rs;
+    for (; r + (U - 1) * rowSplit < nv; r += U * rowSplit)
     {
-      float4 y = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (any)
-      {
-        const float4 x = *reinterpret_cast<const float4 *>(in + inView + size_t(r) * nuIn + (j0 - offset));
-        y = make_float4(Apply<MODE>(x.x, w[0]), Apply<MODE>(x.y, w[1]), Apply<MODE>(x.z, w[2]),
-                        Apply<MODE>(x.w, w[3]));
-      }
-      *reinterpret_cast<float4 *>(out + outView + size_t(r) * nuOut + j0) = y;
+      Vec4 x[U];
+#pragma unroll
+      for (int q = 0; q < U; q++)
+        x[q] = any ? LoadStream(src + size_t(r + q * rowSplit) * nuIn) : Vec4{ 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+      for (int q = 0; q < U; q++)
+        StoreStream(dst + size_t(r + q * rowSplit) * nuOut,
+                    Vec4{ Apply<MODE>(x[q].x, w[0]), Apply<MODE>(x[q].y, w[1]), Apply<MODE>(x[q].z, w[2]),
+                          Apply<MODE>(x[q].w, w[3]) });
+    }
+    for (; r < nv; r += rowSplit)
+    {
+      const Vec4 x = any ? LoadStream(src + size_t(r) * nuIn) : Vec4{ 0.f, 0.f, 0.f, 0.f };
+      StoreStream(dst + size_t(r) * nuOut,
+                  Vec4{ Apply<MODE>(x.x, w[0]), Apply<MODE>(x.y, w[1]), Apply<MODE>(x.z, w[2]), Apply<MODE>(x.w, w[3]) });
     }
   }
   else
