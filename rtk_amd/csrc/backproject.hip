@@ -48,38 +48,57 @@ struct alignas(16) ViewCoef
 // and that use the OPTIMIZED border rule (RTK's OptimizedBackprojectionY case).
 constexpr int kTileU = 64, kTileV = 64, kTileStride = 65; // staged detector footprint, pixels
 
+// Lane exchange inside quads of lanes (DPP quad permutations, no LDS traffic).
+__device__ __forceinline__ float
+QuadXor1(float v)
+{
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true)); // [1,0,3,2]
+}
+__device__ __forceinline__ float
+QuadXor2(float v)
+{
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true)); // [2,3,0,1]
+}
+
 // Detector rectangle [u0, u0 + W) x [v0, v0 + H) holding every tap of the voxel block
 // [xlo, xhi] x [ylo, yhi] x [zlo, zhi] (centred coordinates) for view c.  u, v are
 // linear-fractional in the voxel position; where the denominator w keeps one sign
 // over the block (positive for the FDK's normalised matrices, negative for the plain
 // backprojector's) the block's corners bound them.  One pixel of margin covers
-// rounding.  W = 0: no tap can pass the border rule (skip the view); W < 0: too
-// large, or w changes sign (read global memory for this view).
+// rounding (including the 1-ulp reciprocal).  W = 0: no tap can pass the border rule
+// (skip the view); W < 0: too large, or w changes sign (read global memory for this
+// view).  Called by a whole wavefront: the four lanes of each quad evaluate the four
+// (x, z) corners (lane & 3) and combine them, so every lane returns the rectangle.
 __device__ __forceinline__ void
-Footprint(const ViewCoef & c, float xlo, float xhi, float ylo, float yhi, float zlo, float zhi, int nu, int nv,
-          int & u0, int & v0, int & W, int & H)
+FootprintQuad(const ViewCoef & c, float xlo, float xhi, float ylo, float yhi, float zlo, float zhi, int nu, int nv,
+              int lane, int & u0, int & v0, int & W, int & H)
 {
-  float umin = FLT_MAX, umax = -FLT_MAX, vmin = FLT_MAX, vmax = -FLT_MAX;
-  bool  pos = true, neg = true;
-#pragma unroll
-  for (int q = 0; q < 4; q++)
-  {
-    const float x = (q & 1) ? xhi : xlo, z = (q & 2) ? zhi : zlo;
-    const float w = c.c[8] + c.c[9] * x + c.c[11] * z;
-    pos = pos && (w > 0.f);
-    neg = neg && (w < 0.f);
-    const float rw = 1.f / w;
-    const float u = (c.c[0] + c.c[1] * x + c.c[3] * z) * rw;
-    const float vl = (c.c[4] + c.c[5] * x + c.c[7] * z + c.c[6] * ylo) * rw;
-    const float vh = (c.c[4] + c.c[5] * x + c.c[7] * z + c.c[6] * yhi) * rw;
-    umin = fminf(umin, u);
-    umax = fmaxf(umax, u);
-    vmin = fminf(vmin, fminf(vl, vh));
-    vmax = fmaxf(vmax, fmaxf(vl, vh));
-  }
+  const float x = (lane & 1) ? xhi : xlo, z = (lane & 2) ? zhi : zlo;
+  const float w = c.c[8] + c.c[9] * x + c.c[11] * z;
+  const float rw = __builtin_amdgcn_rcpf(w);
+  const float u = (c.c[0] + c.c[1] * x + c.c[3] * z) * rw;
+  const float vl = (c.c[4] + c.c[5] * x + c.c[7] * z + c.c[6] * ylo) * rw;
+  const float vh = (c.c[4] + c.c[5] * x + c.c[7] * z + c.c[6] * yhi) * rw;
+  float       umin = u, umax = u, vmin = fminf(vl, vh), vmax = fmaxf(vl, vh);
+  // +1 / -1 where w > 0 / w < 0 at this corner, 0 otherwise: the block keeps one
+  // sign iff the quad's minimum and maximum are equal and non-zero
+  float sgn = (w > 0.f) ? 1.f : (w < 0.f ? -1.f : 0.f), smin = sgn, smax = sgn;
+  umin = fminf(umin, QuadXor1(umin));
+  umax = fmaxf(umax, QuadXor1(umax));
+  vmin = fminf(vmin, QuadXor1(vmin));
+  vmax = fmaxf(vmax, QuadXor1(vmax));
+  smin = fminf(smin, QuadXor1(smin));
+  smax = fmaxf(smax, QuadXor1(smax));
+  umin = fminf(umin, QuadXor2(umin));
+  umax = fmaxf(umax, QuadXor2(umax));
+  vmin = fminf(vmin, QuadXor2(vmin));
+  vmax = fmaxf(vmax, QuadXor2(vmax));
+  smin = fminf(smin, QuadXor2(smin));
+  smax = fmaxf(smax, QuadXor2(smax));
   u0 = v0 = 0;
   H = 0;
-  if (!(pos || neg) || !(umin <= umax) || !(vmin <= vmax) || !(umax - umin < 1e6f) || !(vmax - vmin < 1e6f))
+  if (!(smin == smax && smin != 0.f) || !(umin <= umax) || !(vmin <= vmax) || !(umax - umin < 1e6f) ||
+      !(vmax - vmin < 1e6f))
   {
     W = -1;
     return;
@@ -176,16 +195,21 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
   // nu, u0 and sc multiples of 4, a chunk that starts inside the rectangle (sc < W)
   // lies inside its detector row; lanes outside read the view's first 4 pixels
   // instead, which store() discards with the same test.
-  auto  load = [&](int p, int u0, int v0, int W, int H) {
+  // the view's base address and row offsets are wave-uniform (scalar registers);
+  // each lane adds one 32-bit offset (the host keeps a view under 4 GiB)
+  const unsigned laneOff = unsigned(sr * nu + sc); // pixel offset of this lane's chunk in pass 0
+  auto           load = [&](int p, int u0, int v0, int W, int H) {
     const long vbase = long(p) * projStride;
     if constexpr (VEC)
     {
+      const TP * vp = proj + vbase;
+      const bool colIn = sc < W;
 #pragma unroll
       for (int q = 0; q < 4; q++)
       {
-        const int  r = sr + 16 * q;
-        const long o = (r < H && sc < W) ? vbase + long(v0 + r) * nu + (u0 + sc) : vbase;
-        Load4(proj + o, sreg + 4 * q);
+        const unsigned rowOff = unsigned((v0 + 16 * q) * nu + u0); // wave-uniform
+        const unsigned o = (colIn && sr + 16 * q < H) ? rowOff + laneOff : 0u;
+        Load4(vp + o, sreg + 4 * q);
       }
     }
     else
@@ -215,15 +239,18 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
       }
     }
   };
-  // thread 0 places the footprint of view q two views ahead, in rects[q & 1]; the
-  // slot it overwrites was read before the barrier that precedes the write
-  auto publish = [&](int q) {
+  // one wavefront places the footprint of view q two views ahead, in rects[q & 1]
+  // (wave q & 3, so the four waves share the work); the slot it overwrites was read
+  // before the barrier that precedes the write
+  const int wave = t >> 6;
+  auto      publish = [&](int q) {
     int r[4];
-    Footprint(views[q], xlo, xhi, ylo, yhi, zlo, zhi, nu, nv, r[0], r[1], r[2], r[3]);
-    for (int e = 0; e < 4; e++)
-      rects[q & 1][e] = r[e];
+    FootprintQuad(views[q], xlo, xhi, ylo, yhi, zlo, zhi, nu, nv, t & 63, r[0], r[1], r[2], r[3]);
+    if ((t & 63) == 0)
+      for (int e = 0; e < 4; e++)
+        rects[q & 1][e] = r[e];
   };
-  if (t == 0)
+  if (wave == 0)
   {
     publish(0);
     if (nviews > 1)
@@ -249,13 +276,13 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
       nH = rects[(p + 1) & 1][3];
       load(p + 1, nu0, nv0, nW, nH);
     }
-    if (t == 0 && p + 2 < nviews)
+    if (wave == (p & 3) && p + 2 < nviews)
       publish(p + 2);
     if (live && cW != 0)
     {
       const ViewCoef & c = views[p];
       const float      w = c.c[8] + c.c[9] * x + c.c[11] * z;
-      const float      rw = 1.f / w;
+      const float      rw = __builtin_amdgcn_rcpf(w); // 1 ulp
       const float      u = (c.c[0] + c.c[1] * x + c.c[3] * z) * rw;
       const float      uf = floorf(u);
       if (uf >= 0.f && uf < numax)
@@ -283,10 +310,10 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
           for (int jj = 0; jj < NJ; jj++)
           {
             const float   vv = vb + float(jj) * dv;
-            const float   vf = floorf(vv);
-            const float   fv = vv - vf;
+            // vv >= 0 on this path: truncation is the floor, and fract(vv) = vv - floor(vv)
+            const float   fv = __builtin_amdgcn_fractf(vv);
             // 24-bit multiply: the LDS offset is small and the 32 x 32 form is slower
-            const float * s0 = S + (int(__umul24(unsigned(int(vf)), unsigned(kTileStride))) + off);
+            const float * s0 = S + (int(__umul24(unsigned(int(vv)), unsigned(kTileStride))) + off);
             const float   t0 = s0[0] + fv * (s0[kTileStride] - s0[0]);
             const float   t1 = s0[1] + fv * (s0[kTileStride + 1] - s0[1]);
             acc[jj] = fmaf(w1, t1, fmaf(w0, t0, acc[jj]));
@@ -442,7 +469,9 @@ Launch(bool column, const ViewCoef * dcoef, const int * dborder, int nviews, con
   if (column)
   {
     dim3       grid((nx + 15) / 16, (nz + 15) / 16, (ny + kNJ - 1) / kNJ);
-    const bool vec = (nu & 3) == 0 && (reinterpret_cast<size_t>(p) & 15) == 0;
+    // the vector staging path addresses a view with 32-bit byte offsets
+    const bool vec = (nu & 3) == 0 && (reinterpret_cast<size_t>(p) & 15) == 0 &&
+                     size_t(nu) * size_t(nv) * sizeof(TP) < (size_t(1) << 32);
     if (vec)
       hipLaunchKernelGGL((bp_lds_kernel<TP, FDK, kNJ, true>), grid, dim3(256), 0, s, dcoef, nviews, p, nu, nv, vin,
                          vout, nx, ny, nz, ci, cj, ck);
