@@ -46,7 +46,10 @@ struct alignas(16) ViewCoef
 
 // Column kernel, for views whose u and w do not depend on y (c[2] == c[10] == 0)
 // and that use the OPTIMIZED border rule (RTK's OptimizedBackprojectionY case).
-constexpr int kTileU = 64, kTileV = 64, kTileStride = 65; // staged detector footprint, pixels
+constexpr int kTileU = 64, kTileV = 48, kTileStride = 65; // staged detector footprint, pixels
+constexpr int kStagePasses = (kTileV + 15) / 16;           // 16 rows per staging pass
+// 48 rows hold a 16-voxel column's taps up to 2.6 detector rows per voxel (the
+// 512^3 <- 1024^2 FDK reaches 2.34); each extra staging pass costs ~1 VALU per update
 
 // Lane exchange inside quads of lanes (DPP quad permutations, no LDS traffic).
 __device__ __forceinline__ float
@@ -184,10 +187,10 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
   const long  projStride = long(nu) * nv;
   const float numax = float(nu - 1);
   // staging: lane = 4 consecutive pixels (one 16-byte load for FP32 when rows are
-  // 4-aligned), 16 lanes per 64-pixel row, 16 rows per pass, 4 passes
+  // 4-aligned), 16 lanes per 64-pixel row, 16 rows per pass
   const int   sc = 4 * (t & 15), sr = t >> 4;
 
-  float sreg[16];
+  float sreg[4 * kStagePasses];
   int   cu0, cv0, cW, cH; // footprint of the view being backprojected
   // VEC (nu a multiple of 4, 16-byte aligned stack): every lane loads its four 4-pixel
   // chunks, 16 bytes (8 for 16-bit stacks) each, with no branch, so the loaded
@@ -205,7 +208,7 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
       const TP * vp = proj + vbase;
       const bool colIn = sc < W;
 #pragma unroll
-      for (int q = 0; q < 4; q++)
+      for (int q = 0; q < kStagePasses; q++)
       {
         const unsigned rowOff = unsigned((v0 + 16 * q) * nu + u0); // wave-uniform
         const unsigned o = (colIn && sr + 16 * q < H) ? rowOff + laneOff : 0u;
@@ -215,7 +218,7 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
     else
     {
 #pragma unroll
-      for (int q = 0; q < 4; q++)
+      for (int q = 0; q < kStagePasses; q++)
 #pragma unroll
         for (int e = 0; e < 4; e++)
         {
@@ -228,7 +231,7 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
   // no tap reads, and sc + 3 <= 63 keeps them inside the tile
   auto store = [&](int b, int W, int H) {
 #pragma unroll
-    for (int q = 0; q < 4; q++)
+    for (int q = 0; q < kStagePasses; q++)
     {
       const int r = sr + 16 * q;
       if (r < H && sc < W)
