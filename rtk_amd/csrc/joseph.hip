@@ -127,6 +127,7 @@ struct Ray
   int    axis;           // dominant axis (0 x, 1 y, 2 z)
   double cx, cy;         // in-slice position at slice ns
   double stepx, stepy;   // in-slice advance per slice
+  long long sx32, sy32;  // the same in 32.32 fixed point (|step| <= 1)
   double maxx, maxy;     // box limits used by the border clamp
   double wB, wE, wOne;   // first/last step lengths, single-step length when fs == ns
   double stepNorm;       // |stepMM|
@@ -231,6 +232,8 @@ SetupRay(const JosephView & V, int iu, int jv, int p, int nx, int ny, int nz, do
     R.stepx = -R.stepx;
     R.stepy = -R.stepy;
   }
+  R.sx32 = __double2ll_rn(R.stepx * 4294967296.0);
+  R.sy32 = __double2ll_rn(R.stepy * 4294967296.0);
   R.cx = npt[dInf] + residualB * R.stepx;
   R.cy = npt[dSup] + residualB * R.stepy;
   R.wB = residualB + 0.5;
@@ -277,20 +280,24 @@ BorderTaps(const Ray & R, double x, double y, int t[4], float w[4])
     t[q] = tx[q] + ty[q] * R.oy;
 }
 
-// Middle steps: the in-slice position as integer index + FP64 fraction.  Middle
-// samples lie on the segment inside the box; the clamp only guards grazing rays
-// whose rounded position leaves [0, N-2].
+// Middle steps: the in-slice position in 32.32 fixed point, the voxel index in the
+// high word and the fraction in the low word, advanced by one 64-bit integer add per
+// slice (no FP64 arithmetic on the walk).  Rounding the start and the step to 2^-32
+// moves a sample by at most n * 2^-33 voxel after n steps, far below the FP32
+// interpolation weights' resolution.  Middle samples lie on the segment inside the
+// box; the index clamp only guards grazing rays whose rounded position leaves
+// [0, N-2].  The forward and back projectors share this walk, so they stay exact
+// transposes of each other.
 struct Walker
 {
-  int    ix, iy;
-  double fx, fy;
+  long long X, Y;
   __device__ __forceinline__ void Init(double cx, double cy)
   {
-    ix = int(floor(cx));
-    iy = int(floor(cy));
-    fx = cx - ix;
-    fy = cy - iy;
+    X = __double2ll_rn(cx * 4294967296.0);
+    Y = __double2ll_rn(cy * 4294967296.0);
   }
+  __device__ __forceinline__ double PosX() const { return double(X) * (1. / 4294967296.); }
+  __device__ __forceinline__ double PosY() const { return double(Y) * (1. / 4294967296.); }
   __device__ __forceinline__ int Stage(const Ray & R, float & lx, float & ly)
   {
     int cix, ciy;
@@ -300,20 +307,12 @@ struct Walker
   // Lower-left tap (cix, ciy) of the current sample, then advance one slice.
   __device__ __forceinline__ void Stage2D(const Ray & R, int & cix, int & ciy, float & lx, float & ly)
   {
-    cix = min(max(ix, 0), R.nxm2);
-    ciy = min(max(iy, 0), R.nym2);
-    lx = float(fx);
-    ly = float(fy);
-    // fraction in [0, 1) plus a step of magnitude <= 1 lies in [-1, 2): its floor is
-    // -1, 0 or 1 and subtracting it is exact (the index clamp above keeps even a
-    // non-finite fraction in bounds)
-    fx += R.stepx;
-    fy += R.stepy;
-    const double flx = floor(fx), fly = floor(fy);
-    fx -= flx;
-    fy -= fly;
-    ix += int(flx);
-    iy += int(fly);
+    cix = min(max(int(X >> 32), 0), R.nxm2);
+    ciy = min(max(int(Y >> 32), 0), R.nym2);
+    lx = float(unsigned(X)) * 0x1p-32f;
+    ly = float(unsigned(Y)) * 0x1p-32f;
+    X += R.sx32;
+    Y += R.sy32;
   }
 };
 
@@ -404,7 +403,7 @@ __global__ __launch_bounds__(256) void joseph_kernel(const JosephView * __restri
         else if (t <= R.nmid)
           sum += middle(wk, bb);
         else
-          sum += border(R.wE, bb, wk.ix + wk.fx, wk.iy + wk.fy);
+          sum += border(R.wE, bb, wk.PosX(), wk.PosY());
       }
     }
   }
@@ -425,7 +424,7 @@ __global__ __launch_bounds__(256) void joseph_kernel(const JosephView * __restri
         sum += middle(wk, b);
         b += R.oz;
       }
-      sum += border(R.wE, b, wk.ix + wk.fx, wk.iy + wk.fy);
+      sum += border(R.wE, b, wk.PosX(), wk.PosY());
     }
   }
   if (hit)
@@ -483,7 +482,7 @@ SplatRay(const Ray & R, float value, float * acc, float * accT)
     unsafeAtomicAdd(b + idx + 1 + R.oy, s * (lx * ly));
     b += R.oz;
   }
-  border(R.wE, b, wk.ix + wk.fx, wk.iy + wk.fy);
+  border(R.wE, b, wk.PosX(), wk.PosY());
 }
 
 // Tiled splat.  A workgroup is a 16 x 16 tile of detector pixels of one view.  Rays
@@ -706,8 +705,8 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
           else
           {
             step = R.wE;
-            x = wk.ix + wk.fx;
-            y = wk.iy + wk.fy;
+            x = wk.PosX();
+            y = wk.PosY();
           }
           BorderTaps2D(R, x, y, tx, ty, tw);
           sc = float(vn * step);
@@ -738,7 +737,7 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
           const int   a = tx[q] - wx, b = ty[q] - wy;
           const float v = sc * tw[q];
           if (unsigned(a) < unsigned(lim) && unsigned(b) < unsigned(lim))
-            atomicAdd(&w[b * kTileStride + a], ToFixed(v, toFixed));
+            atomicAdd(&w[int(__umul24(unsigned(b), unsigned(kTileStride))) + a], ToFixed(v, toFixed));
           else
             unsafeAtomicAdd(g + tx[q] + ty[q] * oy, v);
         }
