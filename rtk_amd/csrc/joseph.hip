@@ -607,7 +607,9 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
   __syncthreads();
   const int   kmin = krange[0], kmax = krange[1];
   const float maxc = __int_as_float(maxBits);
-  bool        tiled = cornerOk && kmin <= kmax && maxc <= FLT_MAX;
+  // tiles whose largest contribution is below 2^-70 keep the per-ray path: the FP32
+  // flush factor 2^(emax - 50) would leave the normal range
+  bool        tiled = cornerOk && kmin <= kmax && maxc <= FLT_MAX && maxc >= 0x1p-70f;
   auto      window = [&](int k, int & wx0, int & wy0, float & spanx, float & spany) {
     float lox = FLT_MAX, hix = -FLT_MAX, loy = FLT_MAX, hiy = -FLT_MAX;
 #pragma unroll
@@ -657,7 +659,8 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
   // every contribution is at most maxc * (1 + rounding): 2^11 of them stay below 2^61
   int          emax;
   frexp(double(maxc), &emax);
-  const double toFixed = ldexp(1., 50 - emax), fromFixed = ldexp(1., emax - 50);
+  const double toFixed = ldexp(1., 50 - emax);
+  const float  fromFixed32 = ldexpf(1.f, emax - 50);
   Walker       wk;
   int          pwx = 0, pwy = 0; // window origin of the previous slice
   int          plim = kTileWin;  // cells per window edge flushed for the previous slice
@@ -754,13 +757,19 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
       for (int c = tid; c < plim * plim; c += 256)
       {
         const int                i = c & (plim - 1), j = c >> shift;
-        const unsigned long long q = w[j * kTileStride + i];
+        const int                cell = int(__umul24(unsigned(j), unsigned(kTileStride))) + i;
+        const unsigned long long q = w[cell];
         if (q != 0ull)
         {
-          w[j * kTileStride + i] = 0ull;
+          w[cell] = 0ull;
           const int X = pwx + i, Y = pwy + j;
           if (X >= 0 && X < NX && Y >= 0 && Y < NY)
-            unsafeAtomicAdd(g + X + Y * oy, float(double((long long)q) * fromFixed));
+          {
+            // hi * 2^32 + lo in FP32 (one more rounding than an exact conversion, far
+            // below the FP32 atomic's own): no FP64 on the flush
+            const float hi = float(int(q >> 32)), lo = float(unsigned(q));
+            unsafeAtomicAdd(g + X + Y * oy, fmaf(hi, fromFixed32 * 4294967296.f, lo * fromFixed32));
+          }
         }
       }
     }
