@@ -528,7 +528,10 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
                                                               double spy, double spz, double infClip,
                                                               double supClip)
 {
-  __shared__ unsigned long long win[2][kTileWin * kTileStride];
+  // one extra cell per buffer receives the taps that fall outside the window (their
+  // global atomics are issued separately), so the LDS adds need no branch; it is
+  // never flushed
+  __shared__ unsigned long long win[2][kTileWin * kTileStride + 1];
   __shared__ int    vote[6];
   __shared__ int    maxBits; // largest |value * stepNorm| of the participating rays, as FP32 bits
   __shared__ int    krange[2]; // min and max slice over the tile's participating rays
@@ -539,7 +542,7 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
   const int tid = threadIdx.x;
   const int u0 = blockIdx.x * 16, v0 = blockIdx.y * 16, p = blockIdx.z;
   const int iu = u0 + (tid & 15), jv = v0 + (tid >> 4);
-  for (int c = tid; c < 2 * kTileWin * kTileStride; c += 256)
+  for (int c = tid; c < 2 * (kTileWin * kTileStride + 1); c += 256)
     (&win[0][0])[c] = 0ull;
   if (tid < 6)
     vote[tid] = 0;
@@ -734,15 +737,24 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
           tw[3] = lx * ly;
           sc = float(vn);
         }
+        float v[4];
+        int   outside = 0;
 #pragma unroll
         for (int q = 0; q < 4; q++)
         {
-          const int   a = tx[q] - wx, b = ty[q] - wy;
-          const float v = sc * tw[q];
-          if (unsigned(a) < unsigned(lim) && unsigned(b) < unsigned(lim))
-            atomicAdd(&w[int(__umul24(unsigned(b), unsigned(kTileStride))) + a], ToFixed(v, toFixed));
-          else
-            unsafeAtomicAdd(g + tx[q] + ty[q] * oy, v);
+          const int  a = tx[q] - wx, b = ty[q] - wy;
+          const bool in = unsigned(a) < unsigned(lim) && unsigned(b) < unsigned(lim);
+          v[q] = sc * tw[q];
+          atomicAdd(&w[in ? int(__umul24(unsigned(b), unsigned(kTileStride))) + a : kTileWin * kTileStride],
+                    ToFixed(v[q], toFixed));
+          outside |= in ? 0 : 1 << q;
+        }
+        if (outside) // rounding at the window edge: rare
+        {
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            if (outside & (1 << q))
+              unsafeAtomicAdd(g + tx[q] + ty[q] * oy, v[q]);
         }
       }
     }
