@@ -771,17 +771,14 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
         const int                i = c & (plim - 1), j = c >> shift;
         const int                cell = int(__umul24(unsigned(j), unsigned(kTileStride))) + i;
         const unsigned long long q = w[cell];
-        if (q != 0ull)
+        w[cell] = 0ull;
+        const int X = pwx + i, Y = pwy + j;
+        if (q != 0ull && X >= 0 && X < NX && Y >= 0 && Y < NY)
         {
-          w[cell] = 0ull;
-          const int X = pwx + i, Y = pwy + j;
-          if (X >= 0 && X < NX && Y >= 0 && Y < NY)
-          {
-            // hi * 2^32 + lo in FP32 (one more rounding than an exact conversion, far
-            // below the FP32 atomic's own): no FP64 on the flush
-            const float hi = float(int(q >> 32)), lo = float(unsigned(q));
-            unsafeAtomicAdd(g + X + Y * oy, fmaf(hi, fromFixed32 * 4294967296.f, lo * fromFixed32));
-          }
+          // hi * 2^32 + lo in FP32 (one more rounding than an exact conversion, far
+          // below the FP32 atomic's own): no FP64 on the flush
+          const float hi = float(int(q >> 32)), lo = float(unsigned(q));
+          unsafeAtomicAdd(g + X + Y * oy, fmaf(hi, fromFixed32 * 4294967296.f, lo * fromFixed32));
         }
       }
     }
