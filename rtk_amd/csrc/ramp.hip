@@ -252,6 +252,267 @@ __global__ __launch_bounds__(256) void ramp_kernel(const float * __restrict__ in
   }
 }
 
+// ------------------------------------------------------------ register FFTs --
+// Wavefront-per-row variant for M = N/2 in {256, 512, 1024, 2048}: each of the 64
+// lanes holds M/64 complex points in registers; the M-point FFT runs as a mixed-radix
+// Stockham autosort FFT (Govindaraju et al., "High performance discrete Fourier
+// transforms on graphics processors", SC 2008: FftIteration with expand()), whose
+// radix-R butterflies are register DFTs and whose passes exchange data through LDS.
+// One wavefront per workgroup: the barriers between passes cost nothing and a row
+// crosses LDS four times per transform instead of once per radix-4 stage.
+
+constexpr int
+BitRevC(int i, int bits)
+{
+  int r = 0;
+  for (int b = 0; b < bits; b++)
+    r |= ((i >> b) & 1) << (bits - 1 - b);
+  return r;
+}
+
+// R-point DFT (R = 4, 8, 16) of v in registers: radix-2 DIT after a compile-time bit
+// reversal; W_16^m = exp(-2 pi i m / 16), conjugated for the inverse.
+template <int R, bool INV>
+__device__ __forceinline__ void
+DftReg(float2 (&v)[R])
+{
+  constexpr int   bits = R == 4 ? 2 : (R == 8 ? 3 : 4);
+  constexpr float c1 = 0.923879532511286756f, s1 = 0.382683432365089772f, h = 0.707106781186547524f;
+  constexpr float wr[8] = { 1.f, c1, h, s1, 0.f, -s1, -h, -c1 };
+  constexpr float wi[8] = { 0.f, -s1, -h, -c1, -1.f, -c1, -h, -s1 };
+  float2          t[R];
+#pragma unroll
+  for (int i = 0; i < R; i++)
+    t[i] = v[BitRevC(i, bits)];
+#pragma unroll
+  for (int len = 2; len <= R; len <<= 1)
+  {
+#pragma unroll
+    for (int i = 0; i < R; i += len)
+    {
+#pragma unroll
+      for (int k = 0; k < len / 2; k++)
+      {
+        const int m = k * (16 / len);
+        float2    b = t[i + k + len / 2];
+        if (m == 4) // -i (forward), +i (inverse)
+          b = INV ? make_float2(-b.y, b.x) : make_float2(b.y, -b.x);
+        else if (m != 0)
+          b = cmul(b, make_float2(wr[m], INV ? -wi[m] : wi[m]));
+        const float2 a = t[i + k];
+        t[i + k] = make_float2(a.x + b.x, a.y + b.y);
+        t[i + k + len / 2] = make_float2(a.x - b.x, a.y - b.y);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < R; i++)
+    v[i] = t[i];
+}
+
+// One Stockham pass of radix R after passes whose radices multiply to NS: butterfly j
+// reads x[j + r M/R], multiplies by W_{NS R}^{r (j mod NS)} (tw[t] = exp(-2 pi i t/M)),
+// transforms, and writes y[(j - j mod NS) R + j mod NS + r NS].  Each lane does the
+// butterflies j = lane + 64 b; all of them are read before any is written, so the
+// pass can work in place in one LDS buffer.
+template <int M, int R, int NS, bool INV, typename LoadF, typename StoreF>
+__device__ __forceinline__ void
+StockhamPass(LoadF && load, StoreF && store, const float2 * __restrict__ tw, int lane)
+{
+  constexpr int B = M / (R * 64);
+  float2        v[B][R];
+#pragma unroll
+  for (int b = 0; b < B; b++)
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      v[b][r] = load(lane + 64 * b + r * (M / R));
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < B; b++)
+  {
+    const int j = lane + 64 * b, k = j & (NS - 1);
+    if constexpr (NS > 1)
+    {
+#pragma unroll
+      for (int r = 1; r < R; r++)
+      {
+        float2 w = tw[r * k * (M / (NS * R))];
+        if (INV)
+          w.y = -w.y;
+        v[b][r] = cmul(v[b][r], w);
+      }
+    }
+    DftReg<R, INV>(v[b]);
+    const int d = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      store(d + r * NS, v[b][r]);
+  }
+}
+
+template <int M, bool INV, typename LoadF, typename StoreF, typename LdsLoadF, typename LdsStoreF>
+__device__ __forceinline__ void
+StockhamFFT(LoadF && first, StoreF && last, LdsLoadF && lload, LdsStoreF && lstore, const float2 * tw, int lane)
+{
+  if constexpr (M == 256)
+  {
+    StockhamPass<M, 4, 1, INV>(first, lstore, tw, lane);
+    __syncthreads();
+    StockhamPass<M, 4, 4, INV>(lload, lstore, tw, lane);
+    __syncthreads();
+    StockhamPass<M, 4, 16, INV>(lload, lstore, tw, lane);
+    __syncthreads();
+    StockhamPass<M, 4, 64, INV>(lload, last, tw, lane);
+  }
+  else if constexpr (M == 512)
+  {
+    StockhamPass<M, 8, 1, INV>(first, lstore, tw, lane);
+    __syncthreads();
+    StockhamPass<M, 8, 8, INV>(lload, lstore, tw, lane);
+    __syncthreads();
+    StockhamPass<M, 8, 64, INV>(lload, last, tw, lane);
+  }
+  else if constexpr (M == 1024)
+  {
+    StockhamPass<M, 16, 1, INV>(first, lstore, tw, lane);
+    __syncthreads();
+    StockhamPass<M, 16, 16, INV>(lload, lstore, tw, lane);
+    __syncthreads();
+    StockhamPass<M, 4, 256, INV>(lload, last, tw, lane);
+  }
+  else
+  {
+    static_assert(M == 2048, "register FFT sizes: 256, 512, 1024, 2048");
+    StockhamPass<M, 16, 1, INV>(first, lstore, tw, lane);
+    __syncthreads();
+    StockhamPass<M, 16, 16, INV>(lload, lstore, tw, lane);
+    __syncthreads();
+    StockhamPass<M, 8, 256, INV>(lload, last, tw, lane);
+  }
+}
+
+// Same filter as ramp_kernel, one wavefront per (row, view).  tables: [0, M) FFT
+// twiddles exp(-2 pi i t/M), [M, 2M+1) split twiddles exp(-2 pi i k/N), then the
+// kernel K[k] for k in [0, M].
+// TRUNC: the Ohnesorge mirror extends the row into the padding (next > 0); without it
+// a padded sample is in-range data or zero, loaded without branches.
+template <typename TO, int M, bool TRUNC>
+__global__ __launch_bounds__(64) void ramp_wave_kernel(const float * __restrict__ in, TO * __restrict__ out, int nu,
+                                                        int nv, int zeroext, const float2 * __restrict__ tables,
+                                                        int next, const float * __restrict__ mirror,
+                                                        const WeightView * __restrict__ wviews, float dx, float dy)
+{
+  __shared__ float4 buf4[M / 2];
+  float2 *          buf = reinterpret_cast<float2 *>(buf4);
+  const int         lane = threadIdx.x;
+  const int         j = blockIdx.x;
+  const int         p = blockIdx.y;
+  const float *     row = in + (long(p) * nv + j) * nu;
+  TO *              orow = out + (long(p) * nv + j) * nu;
+  const float2 *    tw = tables;
+  const float2 *    split = tables + M;
+  const float2 *    K = split + (M + 1);
+  const WeightView * wv = wviews ? wviews + p : nullptr;
+  // cos(gamma) weighting with the row's constant part hoisted and one reciprocal
+  // square root (1 ulp) per pixel
+  float wx0 = 0.f, wsddtana = 0.f, wnum = 0.f, wsina = 0.f, wfac = 1.f, wC = 0.f;
+  bool  wcos = false;
+  if (wv)
+  {
+    wfac = wv->factor;
+    if (wv->parallel == 0.f)
+    {
+      wcos = true;
+      const float py = wv->y0 + dy * float(j);
+      wx0 = wv->x0;
+      wsddtana = wv->sddtana;
+      wnum = wv->numpart1;
+      wsina = wv->sina;
+      wC = wv->sdd2 + py * py;
+    }
+  }
+  auto W = [&](int i) -> float {
+    if (!wcos)
+      return row[i] * wfac;
+    const float px = wx0 + dx * float(i), t = px - wsddtana;
+    return row[i] * (wfac * (wnum - px * wsina) * __builtin_amdgcn_rsqf(wC + t * t));
+  };
+  auto X = [&](int n) -> float {
+    const int i = n - zeroext;
+    if (!TRUNC)
+      return unsigned(i) < unsigned(nu) ? W(min(max(i, 0), nu - 1)) : 0.f;
+    if (i >= 0 && i < nu)
+      return W(i);
+    {
+      if (i < 0 && -i <= next)
+        return mirror[-i] * (2.f * W(1) - W(-i));
+      if (i >= nu && i - (nu - 1) <= next)
+      {
+        const int b = i - (nu - 1);
+        return mirror[b] * (2.f * W(nu - 1) - W(nu - 1 - b));
+      }
+    }
+    return 0.f;
+  };
+  // the zero-padded, weighted row goes to LDS first (four reals per lane and step),
+  // so the first FFT pass does not hold its register block across global loads
+  for (int q = lane; q < M / 2; q += 64)
+    buf4[q] = make_float4(X(4 * q), X(4 * q + 1), X(4 * q + 2), X(4 * q + 3));
+  __syncthreads();
+  auto lload = [&](int m) { return buf[m]; };
+  auto lstore = [&](int m, float2 v) { buf[m] = v; };
+  StockhamFFT<M, false>(lload, lstore, lload, lstore, tw, lane);
+  __syncthreads();
+
+  // Split to the half spectrum, multiply by the kernel and re-pack, in place: lane
+  // k owns the pair (k, M - k).
+  for (int k = lane; k <= (M >> 1); k += 64)
+  {
+    const int    km = (M - k) & (M - 1);
+    const float2 Zk = buf[k & (M - 1)], Zm = buf[km];
+    auto         Xof = [&](float2 A, float2 Bv, float2 w) {
+      const float2 e = make_float2(0.5f * (A.x + Bv.x), 0.5f * (A.y - Bv.y));
+      const float2 o = make_float2(0.5f * (A.y + Bv.y), -0.5f * (A.x - Bv.x));
+      const float2 wo = cmul(w, o);
+      return make_float2(e.x + wo.x, e.y + wo.y);
+    };
+    float2 Xk, Xm;
+    if (k == 0)
+    {
+      Xk = make_float2(Zk.x + Zk.y, 0.f);
+      Xm = make_float2(Zk.x - Zk.y, 0.f);
+    }
+    else
+    {
+      Xk = Xof(Zk, Zm, split[k]);
+      Xm = Xof(Zm, Zk, split[M - k]);
+    }
+    const float2 Yk = cmul(Xk, K[k]);
+    const float2 Ym = cmul(Xm, K[M - k]);
+    auto         Zof = [&](float2 A, float2 Bv, float2 w) {
+      const float2 e = make_float2(0.5f * (A.x + Bv.x), 0.5f * (A.y - Bv.y));
+      const float2 d = make_float2(0.5f * (A.x - Bv.x), 0.5f * (A.y + Bv.y));
+      const float2 o = cmul(d, conj2(w));
+      return make_float2(e.x - o.y, e.y + o.x);
+    };
+    if (k < M)
+      buf[k] = Zof(Yk, Ym, split[k]);
+    const int mk = M - k;
+    if (mk != k && mk < M)
+      buf[mk] = Zof(Ym, Yk, split[mk]);
+  }
+  __syncthreads();
+  const float scale = 1.f / float(M);
+  auto        gstore = [&](int m, float2 z) {
+    const int i0 = 2 * m - zeroext;
+    if (i0 >= 0 && i0 < nu)
+      StoreOut(orow, i0, z.x * scale);
+    if (i0 + 1 >= 0 && i0 + 1 < nu)
+      StoreOut(orow, i0 + 1, z.y * scale);
+  };
+  StockhamFFT<M, true>(lload, gstore, lload, lstore, tw, lane);
+}
+
 std::vector<WeightView>
 MakeWeightViews(int np, const double * g7, const double origin[2])
 {
@@ -283,6 +544,7 @@ struct RampCache
 {
   std::map<hipStream_t, DeviceBuffer<WeightView>> views;
   std::map<hipStream_t, DeviceBuffer<float2>>     tables;
+  std::map<hipStream_t, DeviceBuffer<float2>>     waveTables;
   std::map<hipStream_t, DeviceBuffer<float>>      mirror;
 };
 thread_local RampCache g_ramp;
@@ -382,6 +644,59 @@ rtkhip_ramp_filter(const int proj_dim[3], int xpad, const double * kernel, int t
       dw = dv.get();
       dx = float(proj_spacing[0]);
       dy = float(proj_spacing[1]);
+    }
+    if (M == 256 || M == 512 || M == 1024 || M == 2048)
+    {
+      // wavefront-per-row register FFT; its own table layout
+      std::vector<float2> wt(M + 2 * (M + 1));
+      for (int t = 0; t < M; t++)
+        wt[t] = make_float2(float(std::cos(-2 * pi * t / M)), float(std::sin(-2 * pi * t / M)));
+      for (int k = 0; k <= M; k++)
+      {
+        wt[M + k] = make_float2(float(std::cos(-2 * pi * k / xpad)), float(std::sin(-2 * pi * k / xpad)));
+        wt[2 * M + 1 + k] = make_float2(float(kernel[2 * k]), float(kernel[2 * k + 1]));
+      }
+      auto & dwt = g_ramp.waveTables[s];
+      dwt.Resize(wt.size());
+      RTKHIP_CHECK(hipMemcpyAsync(dwt.get(), wt.data(), sizeof(float2) * wt.size(), hipMemcpyHostToDevice, s));
+      dim3 wgrid(nv, np);
+#define RTKHIP_RAMPW(TO, MM)                                                                                         \
+  do                                                                                                                 \
+  {                                                                                                                  \
+    if (trunc_next > 0)                                                                                              \
+      hipLaunchKernelGGL((ramp_wave_kernel<TO, MM, true>), wgrid, dim3(64), 0, s, dev_in,                            \
+                         static_cast<TO *>(dev_out), nu, nv, int(zeroext), dwt.get(), trunc_next, dmirror, dw, dx,   \
+                         dy);                                                                                        \
+    else                                                                                                             \
+      hipLaunchKernelGGL((ramp_wave_kernel<TO, MM, false>), wgrid, dim3(64), 0, s, dev_in,                           \
+                         static_cast<TO *>(dev_out), nu, nv, int(zeroext), dwt.get(), trunc_next, dmirror, dw, dx,   \
+                         dy);                                                                                        \
+  } while (0)
+#define RTKHIP_RAMPW_M(TO)                                                                                           \
+  if (M == 256)                                                                                                      \
+    RTKHIP_RAMPW(TO, 256);                                                                                           \
+  else if (M == 512)                                                                                                 \
+    RTKHIP_RAMPW(TO, 512);                                                                                           \
+  else if (M == 1024)                                                                                                \
+    RTKHIP_RAMPW(TO, 1024);                                                                                          \
+  else                                                                                                               \
+    RTKHIP_RAMPW(TO, 2048);
+      if (out_dtype == RTKHIP_F32)
+      {
+        RTKHIP_RAMPW_M(float)
+      }
+      else if (out_dtype == RTKHIP_F16)
+      {
+        RTKHIP_RAMPW_M(__half)
+      }
+      else
+      {
+        RTKHIP_RAMPW_M(__hip_bfloat16)
+      }
+#undef RTKHIP_RAMPW_M
+#undef RTKHIP_RAMPW
+      RTKHIP_CHECK_LAUNCH();
+      return;
     }
     const size_t lds = sizeof(float2) * (2 * M + (3 * M) / 4);
     int          dev = 0, maxLds = 0;

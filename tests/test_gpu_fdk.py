@@ -47,15 +47,21 @@ RAMPS = [dict(), dict(hann=0.8), dict(cosine=0.6), dict(hamming=0.9), dict(ramla
          dict(hann=0.8, truncation=0.1)]
 
 
-@pytest.mark.parametrize("kw", RAMPS, ids=[str(k) for k in RAMPS])
-def test_fdk_weight_plus_ramp(oracle, hip, kw):
+# detector widths: 114 (padded to 256, the LDS FFT), 200 / 300 / 700 / 1500 (padded to
+# 512 / 1024 / 2048 / 4096: the wavefront register FFTs of 256 ... 2048 points)
+RAMP_CASES = [(kw, 114) for kw in RAMPS] + [(kw, 700) for kw in RAMPS] + \
+    [(dict(hann=0.8, truncation=0.1), nu) for nu in (200, 300, 1500)] + [(dict(), 201)]
+
+
+@pytest.mark.parametrize("kw,nu", RAMP_CASES, ids=[f"{k}-{n}" for k, n in RAMP_CASES])
+def test_fdk_weight_plus_ramp(oracle, hip, kw, nu):
     """Fused weighting + ramp via the FDK filter with a zero-contribution check:
     compare the filtered stack through a one-view FDK BP would be indirect, so the
     weighted+ramped stack is compared directly using the kernel-level ABI."""
     from rtk_amd import _lib
     n = 5
     rows = _weight_rows(n)
-    nu, nv = 114, 9  # not a power of two: padded to 256 (greatest prime factor 2)
+    nv = 9  # widths are not powers of two: padded (greatest prime factor 2)
     st = oracle.image((nu, nv, n), (-60., -6., 0.), (1.0, 1.3, 1.))
     rng = np.random.default_rng(6)
     p = (1 + 0.1 * rng.normal(size=(n, nv, nu))).astype(np.float32)
@@ -63,7 +69,7 @@ def test_fdk_weight_plus_ramp(oracle, hip, kw):
     okw = dict(kw)
     ref = oracle.ramp(st, w, gpf=2, **okw)
     xpad, _ = oracle.padded_size(nu, nv, gpf=2)
-    assert xpad == 256
+    assert xpad == 1 << int(np.ceil(np.log2(2 * nu)))
     K = oracle.ramp_kernel(xpad, 1, 1.0, **{k: v for k, v in okw.items() if k != "truncation"})[0]
     kern = np.stack([K.real, K.imag], -1).ravel()
     zeroext = (xpad - nu) // 2
