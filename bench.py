@@ -249,6 +249,45 @@ def warm_start_measurement(rtk, np, torch, seeds, tol):
             "per_seed": rows}
 
 
+def joseph_pair_measurement(rtk, np, torch, reps=5):
+    """The CG operator's projectors at the sparse-view shape: Joseph forward projection
+    of a 20-ellipsoid phantom (256^3 @1 mm) onto 90 x 512^2 @0.8 mm, and the Joseph
+    backprojection (its exact transpose) of the noisy stack.  Kernel-side time from
+    HIP events on the launch stream, best of `reps`."""
+    from rtk_amd import warmstart_train as W
+    g = W.geometry()
+    proj = W.noisy_projections(g, W.EVAL_SEED0)
+    vol = W.volume_source()
+    x = rtk.draw_quadrics(vol, rtk.phantoms.random_ellipsoids(W.EVAL_SEED0, n=20, half_fov=100.))
+    fp = rtk.HipForwardProjectionImageFilter()
+    fp.SetInput(0, proj.like(torch.zeros_like(proj.tensor)))
+    fp.SetInput(1, x)
+    fp.SetGeometry(g)
+    fp.Update(count_samples=True)
+    samples = fp.GetNumberOfSamples()
+    bp = rtk.HipJosephBackProjectionImageFilter()
+    bp.SetInput(0, vol)
+    bp.SetInput(1, proj)
+    bp.SetGeometry(g)
+    bp.Update()
+    out = {"workload": "Joseph FP 256^3 @1 mm -> 90 x 512^2 @0.8 mm and Joseph BP back (the CG operator's pair)",
+           "samples": int(samples)}
+    for name, f in (("joseph_fp", fp), ("joseph_bp", bp)):
+        ms = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            f.Update()
+            e1.record()
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        t = min(ms)
+        out[name] = {"ms": round(t, 3), "gsamples_per_s": round(samples / t / 1e6, 1),
+                     "alg_gb_per_s": round(samples * 16 / t / 1e6, 1)}
+    log(f"joseph pair: {out}")
+    return out
+
+
 def _warm_seeds(args):
     from rtk_amd import warmstart_train as W
     return [W.EVAL_SEED0 + i for i in range(args.warm_seeds)]
@@ -271,6 +310,8 @@ def main():
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--cg-only", action="store_true", help="skip the FDK measurement")
+    ap.add_argument("--joseph-only", action="store_true",
+                    help="only the Joseph FP / BP pair at the sparse-view CG shape")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -291,6 +332,9 @@ def main():
         torch.cuda.synchronize()
         log(f"calibration copy of {args.calib_copy_mib} MiB done")
         del a, b
+    if args.joseph_only:
+        print(json.dumps({"joseph_pair_sparse_view": joseph_pair_measurement(rtk, np, torch)}), flush=True)
+        return
     if args.cg_only:
         print(json.dumps({"cg": cg_measurement(rtk, np, torch, args.cg_iter, args.cg_tol),
                           "cg_warm_start": warm_start_measurement(rtk, np, torch, _warm_seeds(args), args.cg_tol)}),
@@ -353,6 +397,7 @@ def main():
     if not args.no_weights:
         result["short_scan_weighting"] = weighting_measurement(rtk, np, torch, proj)
     if not args.no_cg:
+        result["joseph_pair_sparse_view"] = joseph_pair_measurement(rtk, np, torch)
         result["cg"] = cg_measurement(rtk, np, torch, args.cg_iter, args.cg_tol)
         result["cg_warm_start"] = warm_start_measurement(rtk, np, torch, _warm_seeds(args), args.cg_tol)
     if not args.no_cpu:

@@ -324,9 +324,11 @@ __global__ __launch_bounds__(256) void joseph_kernel(const JosephView * __restri
                                                       double spx, double spy, double spz, double infClip,
                                                       double supClip, unsigned long long * samples)
 {
+  // grid (u tiles, views, v bands): workgroups in flight share a band of detector
+  // rows, whose rays stay in one slab of the volume, so the slab stays in L2
   const int  iu = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int  jv = blockIdx.y * 16 + (threadIdx.x >> 4);
-  const int  p = blockIdx.z;
+  const int  jv = blockIdx.z * 16 + (threadIdx.x >> 4);
+  const int  p = blockIdx.y;
   const bool inside = iu < nu && jv < nv;
   const long pix = (long(p) * nv + jv) * nu + iu;
   long long  cnt = 0;
@@ -540,7 +542,9 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
   __shared__ int    wpar[2][3]; // window origin and edge of the next two slices, by thread 0
 
   const int tid = threadIdx.x;
-  const int u0 = blockIdx.x * 16, v0 = blockIdx.y * 16, p = blockIdx.z;
+  // grid (u tiles, views, v bands), as the forward projector: the band's slab of the
+  // accumulator stays in L2 while all views add to it
+  const int u0 = blockIdx.x * 16, v0 = blockIdx.z * 16, p = blockIdx.y;
   const int iu = u0 + (tid & 15), jv = v0 + (tid >> 4);
   for (int c = tid; c < 2 * (kTileWin * kTileStride + 1); c += 256)
     (&win[0][0])[c] = 0ull;
@@ -860,7 +864,7 @@ rtkhip_joseph_forward_project(const int proj_dim[3], const int vol_dim[3], const
     RTKHIP_CHECK_LAUNCH();
     // Ray clip (.hxx:216-218): understood pixel->source, hence inverted.
     const double infClip = 1. - superior_clip, supClip = 1. - inferior_clip;
-    dim3         grid((nu + 15) / 16, (nv + 15) / 16, np);
+    dim3         grid((nu + 15) / 16, np, (nv + 15) / 16);
     if (samples)
     {
       auto & dc = g_joseph.count[s];
@@ -915,7 +919,7 @@ rtkhip_joseph_backproject(const int proj_dim[3], const int vol_dim[3], const dou
     dT.Resize(nvox);
     RTKHIP_CHECK(hipMemsetAsync(dT.get(), 0, nvox * sizeof(float), s));
     const double infClip = 1. - superior_clip, supClip = 1. - inferior_clip;
-    dim3         grid((nu + 15) / 16, (nv + 15) / 16, np);
+    dim3         grid((nu + 15) / 16, np, (nv + 15) / 16);
 #define RTKHIP_JBP_LAUNCH(TP)                                                                                      \
   hipLaunchKernelGGL(joseph_bp_tile_kernel<TP>, grid, dim3(256), 0, s, dv, nu, nv, static_cast<const TP *>(dev_proj), \
                      dev_vol_out, dT.get(), vol_dim[0], vol_dim[1], vol_dim[2], vol_spacing[0], vol_spacing[1],       \
