@@ -204,6 +204,8 @@ def main():
     ap.add_argument("--cg-iter", type=int, default=100)
     ap.add_argument("--out", default="gpurun_out/warmstart_c4.npz")
     ap.add_argument("--report", default="gpurun_out/warmstart_c4.json")
+    ap.add_argument("--eval-only", action="store_true",
+                    help="evaluate the shipped network on --eval held-out seeds, without training")
     ap.add_argument("--scale-eval", type=int, default=0,
                     help="only evaluate the shipped network at 512^3 / 180 views on this many held-out seeds")
     args = ap.parse_args()
@@ -216,6 +218,18 @@ def main():
                           "eval": rows}), flush=True)
         return
     g = geometry()
+    if args.eval_only:
+        from rtk_amd.warmstart import default_weights_path
+        net = WarmStartCNN.load(default_weights_path())
+        rows = evaluate(net, g, [EVAL_SEED0 + i for i in range(args.eval)])
+        rep = {"workload": "CG 256^3 @1 mm <- 90 x 512^2 @0.8 mm, Poisson I0=1e4, held-out seeds", "eval": rows}
+        for name in ("zero", "fdk", "cnn"):
+            rep[f"median_{name}_seconds"] = float(np.median([r[name]["seconds"] for r in rows]))
+            rep[f"median_{name}_iterations"] = float(np.median([r[name]["iterations"] for r in rows]))
+        rep["speedup_cnn_vs_zero"] = rep["median_zero_seconds"] / rep["median_cnn_seconds"]
+        rep["speedup_fdk_vs_zero"] = rep["median_zero_seconds"] / rep["median_fdk_seconds"]
+        print(json.dumps(rep), flush=True)
+        return
     t0 = time.perf_counter()
     pairs = []
     for i in range(args.train):
