@@ -281,6 +281,9 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
     }
     if (wave == (p & 3) && p + 2 < nviews)
       publish(p + 2);
+    // per-view limits of the fast path (wave-uniform)
+    const bool  viewFast = jn == NJ && cW > 0;
+    const float fvLo = float(cv0), fvHi = float(min(nv - 1, cv0 + cH - 1));
     if (live && cW != 0)
     {
       const ViewCoef & c = views[p];
@@ -302,9 +305,12 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
         // every tap: when both pass the border rule and lie in the staged rectangle,
         // the whole column runs without per-voxel tests
         const float  vlast = vb + float(NJ - 1) * dv;
-        const float  flo = floorf(fminf(vb, vlast)), fhi = floorf(fmaxf(vb, vlast));
-        const bool   fast = jn == NJ && cW > 0 && unsigned(lc) < unsigned(cW - 1) && flo >= 0.f &&
-                          fhi < float(nv - 1) && int(flo) >= cv0 && int(fhi) - cv0 < cH - 1;
+        const float  vmn = vb < vlast ? vb : vlast, vmx = vb < vlast ? vlast : vb;
+        // one combined test (no short-circuit branches): the column lies inside the
+        // staged rows [cv0, cv0 + cH - 1) and inside [0, nv - 1), and its detector
+        // column pair inside the staged columns; cv0 >= 0 by construction
+        const bool   fast = viewFast & (unsigned(lc) < unsigned(cW - 1)) & (floorf(vmn) >= fvLo) &
+                          (floorf(vmx) < fvHi);
         if (fast)
         {
           const int    off = lc - cv0 * kTileStride;
