@@ -810,6 +810,8 @@ CheckArgs(const int proj_dim[3], const int vol_dim[3], const double * proj_to_vo
   RTKHIP_REQUIRE(proj_dim[0] > 0 && proj_dim[1] > 0 && proj_dim[2] >= 0 && vol_dim[0] > 0 && vol_dim[1] > 0 &&
                    vol_dim[2] > 0,
                  RTKHIP_ERR_INVALID_ARGUMENT, "joseph: sizes must be positive");
+  RTKHIP_REQUIRE(vol_dim[0] >= 2 && vol_dim[1] >= 2 && vol_dim[2] >= 2, RTKHIP_ERR_UNSUPPORTED,
+                 "joseph: every volume dimension must be at least 2 (bilinear taps)");
   RTKHIP_REQUIRE(proj_dim[2] <= 65535, RTKHIP_ERR_UNSUPPORTED, "joseph: at most 65535 views per call");
   RTKHIP_REQUIRE(vol_dim[1] <= 65535, RTKHIP_ERR_UNSUPPORTED, "joseph: at most 65535 volume rows");
   const long long nvox = (long long)vol_dim[0] * vol_dim[1] * vol_dim[2];

@@ -101,3 +101,22 @@ def test_no_device_is_a_status():
     m, mp = _lib.dbl(np.zeros(12))
     rc = L.rtkhip_fdk_backproject(pd, pd, mp, None, None, ctypes.c_void_p(8), ctypes.c_void_p(8), 0, None)
     assert rc == _lib.RTKHIP_ERR_NO_DEVICE
+
+
+def test_joseph_rejects_degenerate_volumes_and_too_many_views():
+    """Host checks before any device work: a Joseph walk needs two voxels along every
+    axis for its bilinear taps, and one launch takes at most 65535 views."""
+    from rtk_amd import _lib
+    L = _lib.lib()
+    m, mp = _lib.dbl(np.zeros(12))
+    src, sp = _lib.dbl(np.zeros(3))
+    vs, vsp = _lib.dbl(np.ones(3))
+    dummy = ctypes.c_void_p(256)
+    for vol, views, msg in [((1, 8, 8), 2, b"at least 2"), ((8, 8, 1), 2, b"at least 2"),
+                            ((8, 8, 8), 70000, b"65535")]:
+        rc = L.rtkhip_joseph_forward_project(_lib.int3((4, 4, views)), _lib.int3(vol), mp, sp, vsp, 0., 1., dummy,
+                                             dummy, dummy, None, None)
+        assert rc == _lib.RTKHIP_ERR_UNSUPPORTED and msg in L.rtkhip_last_error()
+        rc = L.rtkhip_joseph_backproject(_lib.int3((4, 4, views)), _lib.int3(vol), mp, sp, vsp, 0., 1., dummy,
+                                         dummy, dummy, _lib.F32, None)
+        assert rc == _lib.RTKHIP_ERR_UNSUPPORTED and msg in L.rtkhip_last_error()
