@@ -15,8 +15,9 @@
 //    v_mfma_f32_16x16x32_bf16: M = 16 voxels along x, N = 16 output channels,
 //    K = 32 input channels per tap, 27 taps accumulated in FP32.  A workgroup is
 //    16 (x) x 4 (y) x 4 (z) voxels; wave w owns z-plane w and four M-tiles (y rows).
-//    A fragments are 16-byte channel chunks of neighbouring voxels (coalesced
-//    1 KB per instruction), B fragments 16-byte chunks of the weights.  Bias, ReLU
+//    The block's input with its one-voxel halo is staged in LDS once; A fragments
+//    are 16-byte channel chunks of neighbouring voxels read from there (1 KB per
+//    instruction, conflict-free), B fragments 16-byte chunks of the weights.  Bias, ReLU
 //    and the ReLU-derivative mask of the data gradient are fused into the epilogue.
 //  * The 1->32 and 32->1 layers and all weight gradients run on the VALU in FP32
 //    (they are 64x smaller or reduction-shaped); weight gradients reduce per block in
@@ -100,6 +101,11 @@ __global__ __launch_bounds__(256) void conv32_mfma_kernel(const bf16_t * __restr
                                                            const bf16_t * __restrict__ mask, bf16_t * __restrict__ out,
                                                            int X, int Y, int Z, int tilesX, int tilesY, int tilesZ)
 {
+  // The block's input with a one-voxel halo, 18 (x) x 6 (y) x 6 (z) voxels of 32
+  // channels, is staged in LDS once (zero outside the volume); every tap's A
+  // fragment is then a conflict-free 1 KB LDS read instead of a global load.
+  constexpr int HX = 18, HY = 6, HZ = 6, NCHUNK = HX * HY * HZ * 4;
+  __shared__ bf16x8 halo[NCHUNK];
   const int  lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   long       blk = blockIdx.x;
   const int  tx = int(blk % tilesX);
@@ -110,9 +116,22 @@ __global__ __launch_bounds__(256) void conv32_mfma_kernel(const bf16_t * __restr
   const int  b = int(blk / tilesZ);
   const int  z = tz * 4 + wave;
   const int  y0 = ty * 4;
-  const int  xa = tx * 16 + (lane & 15); // voxel of this lane's A row
-  const int  kg = lane >> 4;             // channel group: ci = 8 kg .. 8 kg + 7
+  const int  xl = lane & 15;  // voxel of this lane's A row (x offset in the block)
+  const int  kg = lane >> 4;  // channel group: ci = 8 kg .. 8 kg + 7
   const long volBase = long(b) * Z * Y * X;
+
+  const bf16x8 zero8 = bf16x8{ bf16_t(0.f), bf16_t(0.f), bf16_t(0.f), bf16_t(0.f),
+                               bf16_t(0.f), bf16_t(0.f), bf16_t(0.f), bf16_t(0.f) };
+  for (int c = threadIdx.x; c < NCHUNK; c += 256)
+  {
+    const int part = c & 3, vi = c >> 2;
+    const int xs = tx * 16 - 1 + vi % HX, ys = y0 - 1 + (vi / HX) % HY, zs = tz * 4 - 1 + vi / (HX * HY);
+    bf16x8    v = zero8;
+    if (xs >= 0 && xs < X && ys >= 0 && ys < Y && zs >= 0 && zs < Z)
+      v = *reinterpret_cast<const bf16x8 *>(in + ((volBase + (long(zs) * Y + ys) * X + xs) * C + 8 * part));
+    halo[c] = v;
+  }
+  __syncthreads();
 
   floatx4 acc[4][2];
 #pragma unroll
@@ -120,22 +139,16 @@ __global__ __launch_bounds__(256) void conv32_mfma_kernel(const bf16_t * __restr
     for (int nn = 0; nn < 2; nn++)
       acc[r][nn] = floatx4{ 0.f, 0.f, 0.f, 0.f };
 
-  const bf16x8 zero8 = bf16x8{ bf16_t(0.f), bf16_t(0.f), bf16_t(0.f), bf16_t(0.f),
-                               bf16_t(0.f), bf16_t(0.f), bf16_t(0.f), bf16_t(0.f) };
   for (int t = 0; t < T; t++)
   {
     const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
     const bf16x8 b0 = *reinterpret_cast<const bf16x8 *>(w + (long(t) * C + (lane & 15)) * C + 8 * kg);
     const bf16x8 b1 = *reinterpret_cast<const bf16x8 *>(w + (long(t) * C + 16 + (lane & 15)) * C + 8 * kg);
-    const int    zs = z + dz, xs = xa + dx;
-    const bool   inZX = zs >= 0 && zs < Z && xs >= 0 && xs < X;
+    const int    hbase = ((wave + dz + 1) * HY + dy + 1) * HX + xl + dx + 1;
 #pragma unroll
     for (int r = 0; r < 4; r++)
     {
-      const int ys = y0 + r + dy;
-      bf16x8    a = zero8;
-      if (inZX && ys >= 0 && ys < Y)
-        a = *reinterpret_cast<const bf16x8 *>(in + ((volBase + (long(zs) * Y + ys) * X + xs) * C + 8 * kg));
+      const bf16x8 a = halo[(hbase + r * HX) * 4 + kg];
       acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[r][0], 0, 0, 0);
       acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[r][1], 0, 0, 0);
     }
