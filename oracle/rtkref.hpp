@@ -20,8 +20,11 @@
 //   CG              SPD invariants, monotone residual, double vs float
 //   phantoms        Shepp-Logan table src/rtkSheppLoganPhantom.cxx:23-35,
 //                   chord length of a sphere in closed form
-//   ITK generic linear interpolation (rtkBackProjectionImageFilter.hxx:250-277,
-//   used only by non-circular geometries): parity unpinned.
+//   truncation      padding recovered from filtered unit rows by least squares
+//   mirror          (Ohnesorge eqs. 3a/3b, RTK's weights); thresholds of
+//                   test/rtkrampfiltertest.cxx:121-134 pass with it, fail without
+//   ITK generic     brute force with ITK's border (index clamped to [0, n - 1]
+//   interpolation   inside [-0.5, n - 0.5)) on out-of-plane/in-plane geometries
 #pragma once
 
 #include <algorithm>
@@ -400,8 +403,9 @@ TransposedProjection(const T * stack, const ImageInfo & s, int iProj)
 }
 
 // ITK LinearInterpolateImageFunction on a 2-D image (generic BP path,
-// include/rtkBackProjectionImageFilter.hxx:250-277).  Parity unpinned: ITK
-// is not available; restated from ITK 5.4's published 2-D EvaluateOptimized.
+// include/rtkBackProjectionImageFilter.hxx:250-277).  ITK is not available;
+// restated from ITK 5.4's published 2-D EvaluateOptimized: inside the buffer
+// ([-0.5, n - 0.5) per axis) it is bilinear at the index clamped to [0, n - 1].
 template <class T>
 struct LinearInterp2D
 {

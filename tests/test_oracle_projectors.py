@@ -5,7 +5,11 @@ coordinates (project each voxel centre with the documented 3x4 matrix, convert
 to a detector index, bilinear with the CPU border rule) must agree with the
 oracle, which follows RTK's transposed index-to-index matrices and incremental
 Y-path loops.  Constant-1 projections give per-voxel view counts (plain) and
-sums of w^2 (FDK).
+sums of w^2 (FDK).  Geometries with out-of-plane and in-plane angles take RTK's
+generic path (include/rtkBackProjectionImageFilter.hxx:250-277), which samples
+with ITK's linear interpolator: there the brute force accepts continuous indices
+in [-0.5, n - 0.5) and interpolates bilinearly at the index clamped to [0, n - 1]
+(edge pixels extend by a half pixel), a different border than the CPU fast paths.
 
 Joseph: a constant-1 volume gives the exact chord of the voxel-centre box,
 clipped to the pixel-source segment (inner and outer sources); the Shepp-Logan
@@ -18,7 +22,7 @@ from helpers import check_image_quality, circular_rows
 from test_oracle_geometry import doc_matrix
 
 
-def naive_bp(rows, vsize, vorig, vsp, ssize, sorig, ssp, proj, fdk):
+def naive_bp(rows, vsize, vorig, vsp, ssize, sorig, ssp, proj, fdk, itk_border=False):
     nx, ny, nz = vsize
     k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
     X = np.stack([vorig[0] + vsp[0] * i, vorig[1] + vsp[1] * j, vorig[2] + vsp[2] * k, np.ones_like(i, float)], -1)
@@ -29,10 +33,14 @@ def naive_bp(rows, vsize, vorig, vsp, ssize, sorig, ssp, proj, fdk):
         h = X @ M.T
         u = ((h[..., 0] / h[..., 2]) - sorig[0]) / ssp[0]
         v = ((h[..., 1] / h[..., 2]) - sorig[1]) / ssp[1]
+        if itk_border:
+            ok = (u >= -0.5) & (u < nu - 0.5) & (v >= -0.5) & (v < nv - 0.5)
+            u, v = np.clip(u, 0, nu - 1), np.clip(v, 0, nv - 1)
         ui, vi = np.floor(u).astype(int), np.floor(v).astype(int)
-        ok = (ui >= 0) & (ui < nu - 1) & (vi >= 0) & (vi < nv - 1)
+        if not itk_border:
+            ok = (ui >= 0) & (ui < nu - 1) & (vi >= 0) & (vi < nv - 1)
         uc, vc = np.clip(ui, 0, nu - 2), np.clip(vi, 0, nv - 2)
-        fu, fv = u - ui, v - vi
+        fu, fv = u - uc, v - vc
         P = proj[p]
         val = ((1 - fu) * (1 - fv) * P[vc, uc] + fu * (1 - fv) * P[vc, uc + 1] + (1 - fu) * fv * P[vc + 1, uc]
                + fu * fv * P[vc + 1, uc + 1])
@@ -63,6 +71,28 @@ def test_backproject_matches_brute_force(oracle, fdk, gname):
     st = oracle.image(ssize, sorig + (0.,), ssp + (1.,))
     got = oracle.backproject(rows, vol, np.zeros((9, 11, 13)), st, proj, fdk=fdk)
     ref = naive_bp(rows, vsize, vorig, vsp, ssize, sorig, ssp, proj, fdk)
+    np.testing.assert_allclose(got, ref, rtol=1e-9, atol=1e-9)
+
+
+@pytest.mark.parametrize("fdk", [False, True])
+@pytest.mark.parametrize("nu,nv", [(21, 17), (7, 6)])
+def test_backproject_generic_path_matches_brute_force(oracle, fdk, nu, nv):
+    """Out-of-plane (-8..8 deg) and in-plane (5..25 deg) angles: RTK's generic path
+    with ITK's linear interpolation.  The volume overhangs the detector so that
+    voxels land in the half-pixel border bands, where the two border rules differ;
+    the 7 x 6 detector puts samples in the corner bands too."""
+    rows = np.array([[600., 1000., 40 * i + 3, 2. - i, 1.5, 2. * i - 8., 5. + 2.5 * i, 4., -3.]
+                     for i in range(9)])
+    vsize, vsp, vorig = (17, 15, 13), (3., 2.5, 3.5), (-26., -18., -21.)
+    ssize, ssp, sorig = (nu, nv, len(rows)), (4., 4.), (-2. * (nu - 1) + 2., -2. * (nv - 1) - 1.)
+    rng = np.random.default_rng(8)
+    proj = rng.normal(size=(len(rows), nv, nu))
+    vol = oracle.image(vsize, vorig, vsp)
+    st = oracle.image(ssize, sorig + (0.,), ssp + (1.,))
+    got = oracle.backproject(rows, vol, np.zeros(vsize[::-1]), st, proj, fdk=fdk)
+    ref = naive_bp(rows, vsize, vorig, vsp, ssize, sorig, ssp, proj, fdk, itk_border=True)
+    fast_rule = naive_bp(rows, vsize, vorig, vsp, ssize, sorig, ssp, proj, fdk)
+    assert np.max(np.abs(ref - fast_rule)) > 0.1
     np.testing.assert_allclose(got, ref, rtol=1e-9, atol=1e-9)
 
 

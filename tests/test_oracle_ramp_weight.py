@@ -6,11 +6,17 @@
   that is not a power of two (greatest prime factor 13, the FFTW configuration).
 * The Hann window law on the kernel spectrum.
 * FDK weighting against the closed-form cosine of the ray angle.
+* The truncation mirror (Ohnesorge et al., Med Phys 2000, eqs. 3a/3b, with
+  RTK's reference pixels and weights, include/rtkFFTProjectionsConvolutionImageFilter.hxx:
+  213-260,345-355): the padded values are recovered from the filtered output by
+  least squares through the numpy convolution matrix, and the reconstruction
+  thresholds of test/rtkrampfiltertest.cxx:121-134 hold with the mirror and fail
+  without it.
 """
 import numpy as np
 import pytest
 
-from helpers import circular_rows
+from helpers import check_image_quality, circular_rows
 
 
 @pytest.mark.parametrize("kw,expected", [({}, 0.132652), ({"hann": 0.8}, 0.149724),
@@ -37,6 +43,62 @@ def test_ramp_matches_numpy_fft_non_pow2(oracle):
     pad[..., z:z + nu] = p
     ref = np.fft.irfft(np.fft.rfft(pad, axis=-1) * K, n=xpad, axis=-1)[..., z:z + nu]
     assert np.max(np.abs(out - ref)) < 1e-12 * np.max(np.abs(ref)) * 100
+
+
+@pytest.mark.parametrize("nu,trunc", [(40, 0.25), (32, 0.7), (31, 0.2)])
+def test_truncation_mirror_recovered_padding(oracle, nu, trunc):
+    """The ramp filter is linear, so the mirrored padding is the difference of the
+    filtered unit rows with and without the mirror, pulled back through the
+    convolution matrix of the padded positions (numpy FFT of the pinned kernel).
+    The recovered padding must be w_b (2 p_1 - p_b) on the left and
+    w_b (2 p_{n-1} - p_{n-1-b}) on the right, b = 1..next, with
+    w_b = sin((E - b) pi / (2E - 2))^0.75, E = floor(trunc * nu) and next = min(E,
+    the zero-padding width) -- 0.7 * 32 exceeds that width (16)."""
+    xpad, _ = oracle.padded_size(nu, 1)
+    z = (xpad - nu) // 2
+    E = int(np.floor(trunc * nu))
+    nxt = min(E, z)
+    assert (nxt < E) == (trunc == 0.7)
+    st = oracle.image((nu, 1, nu), (0, 0, 0), (1, 1, 1))
+    eye = np.eye(nu)[:, None, :]  # view j holds a unit pixel at column j
+    diff = (oracle.ramp(st, eye, truncation=trunc) - oracle.ramp(st, eye))[:, 0, :].T  # (out i, input j)
+    k = np.fft.irfft(oracle.ramp_kernel(xpad, 1, 1.0)[0], n=xpad)
+    left = [z - b for b in range(1, nxt + 1)]
+    right = [z + nu - 1 + b for b in range(1, nxt + 1)]
+    i = np.arange(nu)[:, None]
+    cols = np.array(left + right)[None, :]
+    A = k[(z + i - cols) % xpad]  # output i <- padded position q
+    assert np.linalg.cond(A) < 1e9
+    pad, *_ = np.linalg.lstsq(A, diff, rcond=None)
+    w = np.sin((E - np.arange(E + 1)) * np.pi / (2 * E - 2)) ** 0.75
+    expect = np.zeros((2 * nxt, nu))
+    for b in range(1, nxt + 1):
+        expect[b - 1, 1] += 2 * w[b]
+        expect[b - 1, b] -= w[b]
+        expect[nxt + b - 1, nu - 1] += 2 * w[b]
+        expect[nxt + b - 1, nu - 1 - b] -= w[b]
+    assert np.max(np.abs(A @ pad - diff)) < 1e-10
+    assert np.max(np.abs(pad - expect)) < 1e-8
+
+
+def test_truncation_mirror_reference_thresholds(oracle):
+    """test/rtkrampfiltertest.cxx:120-134: Shepp-Logan (scale 128) from 180 views on
+    a detector cropped to 114 of 128 columns (4 mm, origin -254), SID 600, SDD
+    1200, 128^3 at 2 mm; with a truncation correction of 0.1 the FDK volume is
+    within 0.12 per voxel and 20.8 dB of the drawn phantom, and it is not
+    without the correction."""
+    from rtk_amd import phantoms
+    n = 180
+    rows = np.array([[600., 1200., i * 360. / n, 0, 0, 0, 0, 0, 0] for i in range(n)])
+    q = phantoms.shepp_logan(128.)
+    vol = oracle.image((128, 128, 128), (-127,) * 3, (2,) * 3)
+    ref = oracle.draw_quadrics(vol, np.zeros((128, 128, 128), np.float32), q)
+    st = oracle.image((114, 128, n), (-254, -254, -254), (4, 4, 4))
+    proj = oracle.project_quadrics(rows, st, np.zeros((n, 128, 114), np.float32), q)
+    res = {t: check_image_quality(oracle.fdk(rows, vol, np.zeros((128, 128, 128), np.float32), st, proj,
+                                             truncation=t), ref, 0.12, 20.8, 2.) for t in (0.1, 0.)}
+    assert res[0.1][2], res[0.1]
+    assert not res[0.][2], res[0.]
 
 
 def test_ramp_kernel_spectrum_and_hann_window(oracle):
