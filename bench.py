@@ -36,7 +36,7 @@ SEED = 20260001
 # HBM bytes per launch of the FDK backprojection kernel on this workload (FP32 stack):
 # rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB device copy),
 # profiles/bp_fdk_512_720_pmc.md.  Re-measure when that kernel changes.
-BP_TRAFFIC_BYTES = 1.536e11
+BP_TRAFFIC_BYTES = 1.543e11
 
 
 def log(msg):
