@@ -46,3 +46,16 @@ def to_device(rtk, arr, origin, spacing, index=(0, 0, 0), dtype=None):
     if dtype is not None:
         t = t.to(dtype)
     return rtk.Image(t, origin, spacing, index)
+
+
+def row_band(oracle, rows, lo, hi, v_origin, v_spacing, nv, margin=2):
+    """Detector rows [r0, r1) that points of the box [lo, hi] (physical x, y, z) reach
+    in any view, plus `margin` rows on each side.  A central projection maps the box
+    into the convex hull of its corners' images, so the corners bound the rows."""
+    m, *_ = oracle.geometry(rows)
+    c = np.array([[x, y, z, 1.] for x in (lo[0], hi[0]) for y in (lo[1], hi[1]) for z in (lo[2], hi[2])])
+    h = np.einsum("prc,nc->pnr", m, c)
+    v = h[..., 1] / h[..., 2]
+    r0 = int(np.floor((v.min() - v_origin) / v_spacing)) - margin
+    r1 = int(np.ceil((v.max() - v_origin) / v_spacing)) + 1 + margin
+    return max(r0, 0), min(r1, nv)

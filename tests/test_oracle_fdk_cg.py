@@ -8,10 +8,14 @@ integration tests and CG identities.
 * CG: test/rtkconjugategradientreconstructiontest.cxx:99-118 (ellipsoid of
   axis 90, 5 iterations, Joseph FP + voxel BP), thresholds 0.08 / 23 dB.
 * CG identity: X_1 = alpha_0 B with alpha_0 = B.B / B.(A B) from X_0 = 0.
+* Row-band restriction (used by the full-size GPU parity tests): the FDK of a few
+  y slices from only the detector rows they reach equals the same slices of the
+  FDK from the whole stack, to the rounding of the pixel coordinates (the band's
+  origin moves by r0 rows); narrowing the band changes them.
 """
 import numpy as np
 
-from helpers import check_image_quality
+from helpers import check_image_quality, row_band
 
 
 def test_fdk_shepp_logan(oracle):
@@ -63,3 +67,29 @@ def test_cg_first_step_identity(oracle):
     np.testing.assert_allclose(res[0], (b * b).sum(), rtol=1e-12)
     r1 = b - alpha * ab
     np.testing.assert_allclose(res[1], (r1 * r1).sum(), rtol=1e-9)
+
+
+def test_fdk_row_band_equals_full_stack(oracle):
+    from rtk_amd import phantoms
+    nx, nu, n, vsp, psp = 64, 128, 45, 4., 3.2
+    vorg, porg = -(nx - 1) * vsp / 2, -(nu - 1) * psp / 2
+    rng = np.random.default_rng(4)
+    offs = rng.uniform(-2., 2., size=(n, 2))
+    rows = np.array([[1000., 1536., i * 360. / n, offs[i, 0], offs[i, 1], 0, 0, 0, 0] for i in range(n)])
+    q = phantoms.random_ellipsoids(4, n=20, half_fov=nx * vsp / 2)
+    st = oracle.image((nu, nu, n), (porg, porg, 0.), (psp, psp, 1.))
+    proj = oracle.project_quadrics(rows, st, np.zeros((n, nu, nu), np.float32), q)
+    full = oracle.fdk(rows, oracle.image((nx,) * 3, (vorg,) * 3, (vsp,) * 3), np.zeros((nx,) * 3, np.float32), st,
+                      proj)
+    for j in (0, 31, 50):
+        y0 = vorg + j * vsp
+        r0, r1 = row_band(oracle, rows, (vorg, y0, vorg), (-vorg, y0 + vsp, -vorg), porg, psp, nu)
+        assert r1 - r0 < nu // 2
+        outs = []
+        for a, b in ((r0, r1), (r0 + 3, r1 - 3)):
+            sb = oracle.image((nu, b - a, n), (porg, porg + a * psp, 0.), (psp, psp, 1.))
+            outs.append(oracle.fdk(rows, oracle.image((nx, 2, nx), (vorg, y0, vorg), (vsp,) * 3),
+                                   np.zeros((nx, 2, nx), np.float32), sb, np.ascontiguousarray(proj[:, a:b])))
+        scale = np.abs(full).max()
+        assert np.abs(outs[0] - full[:, j:j + 2]).max() <= 1e-7 * scale
+        assert np.abs(outs[1] - full[:, j:j + 2]).max() > 1e-3 * scale
