@@ -504,10 +504,14 @@ SplatRay(const Ray & R, float value, float * acc, float * accT)
 // of the tile: a central projection maps the pixel rectangle onto a convex
 // quadrilateral of the slice plane, so the corners bound every ray's position; its
 // width is a convex function of k, so checking the first and last slice bounds all.
-// Slices whose footprint fits 16 x 16 cells use (and flush) only that corner of the
-// window, one cell per lane.  Taps that still fall outside (rounding), rays of a
+// The placements of 256 slices are computed at once, one slice per lane, while the
+// previous 256 slices are walked.  Slices whose footprint fits 16 x 16 cells use (and
+// flush) only that corner of the window, one cell per lane.  A middle step's four
+// taps are a 2 x 2 block of cells: one window test and one LDS address, the other
+// three taps at fixed offsets.  Taps that still fall outside (rounding), rays of a
 // minority axis or direction, and tiles whose footprint exceeds the window or that
-// hold a non-finite value take the per-ray path.
+// hold a non-finite value take the per-ray path.  Every tap lies inside the volume,
+// so a cell with a non-zero sum is a voxel of the slice.
 // round(v * scale) as a two's-complement 64-bit integer, for |v * scale| < 2^51:
 // adding 1.5 * 2^52 leaves the rounded integer (round to nearest even, as
 // __double2ll_rn) in the low mantissa bits, and subtracting the constant's bit
@@ -522,6 +526,9 @@ ToFixed(float v, double scale)
 
 constexpr int kTileWin = 32;    // LDS window edge, in voxels
 constexpr int kTileStride = 33; // LDS row stride, in cells
+constexpr int kTileCells = kTileWin * kTileStride;
+constexpr int kTileDummy = kTileStride + 2; // cells after the window that absorb outside taps
+constexpr int kPlaceChunk = 256;            // slices placed per pass, one per lane
 
 template <typename TP>
 __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * __restrict__ views, int nu, int nv,
@@ -530,23 +537,23 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
                                                               double spy, double spz, double infClip,
                                                               double supClip)
 {
-  // one extra cell per buffer receives the taps that fall outside the window (their
-  // global atomics are issued separately), so the LDS adds need no branch; it is
-  // never flushed
-  __shared__ unsigned long long win[2][kTileWin * kTileStride + 1];
+  // cells after the window receive the taps that fall outside it (their global
+  // atomics are issued separately), so the LDS adds need no branch; they are never
+  // flushed
+  __shared__ unsigned long long win[2][kTileCells + kTileDummy];
   __shared__ int    vote[6];
   __shared__ int    maxBits; // largest |value * stepNorm| of the participating rays, as FP32 bits
   __shared__ int    krange[2]; // min and max slice over the tile's participating rays
   __shared__ float  line[4][4]; // corner c: in-slice x = line[c][0] + k line[c][1], y likewise
   __shared__ int    cornerOk;
-  __shared__ int    wpar[2][3]; // window origin and edge of the next two slices, by thread 0
+  __shared__ int4   wnd[2][kPlaceChunk]; // window origin and edge per slice, two chunks
 
   const int tid = threadIdx.x;
   // grid (u tiles, views, v bands), as the forward projector: the band's slab of the
   // accumulator stays in L2 while all views add to it
   const int u0 = blockIdx.x * 16, v0 = blockIdx.z * 16, p = blockIdx.y;
   const int iu = u0 + (tid & 15), jv = v0 + (tid >> 4);
-  for (int c = tid; c < 2 * (kTileWin * kTileStride + 1); c += 256)
+  for (int c = tid; c < 2 * (kTileCells + kTileDummy); c += 256)
     (&win[0][0])[c] = 0ull;
   if (tid < 6)
     vote[tid] = 0;
@@ -575,6 +582,7 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
   int best = 0;
   for (int q = 1; q < 6; q++)
     best = vote[q] > vote[best] ? q : best;
+  best = __builtin_amdgcn_readfirstlane(best); // workgroup-uniform: scalar registers
   const int  axis = best >> 1, dirSign = (best & 1) ? -1 : 1;
   const bool part = active && key == best;
   if (part)
@@ -612,7 +620,7 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
     }
   }
   __syncthreads();
-  const int   kmin = krange[0], kmax = krange[1];
+  const int   kmin = __builtin_amdgcn_readfirstlane(krange[0]), kmax = __builtin_amdgcn_readfirstlane(krange[1]);
   const float maxc = __int_as_float(maxBits);
   // tiles whose largest contribution is below 2^-70 keep the per-ray path: the FP32
   // flush factor 2^(emax - 50) would leave the normal range
@@ -671,16 +679,20 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
   Walker       wk;
   int          pwx = 0, pwy = 0; // window origin of the previous slice
   int          plim = kTileWin;  // cells per window edge flushed for the previous slice
-  // thread 0 places the window of slice s + 1 while slice s is processed, in
-  // wpar[(s + 1) & 1]; that slot was last read before the barrier ending slice s - 1.
-  // The window edge is 16 when the slice footprint fits.
-  auto place = [&](int k, int * out) {
-    float sx, sy;
-    window(k, out[0], out[1], sx, sy);
-    out[2] = (sx + 5.f <= 16.f && sy + 5.f <= 16.f) ? 16 : kTileWin;
+  // Lane l places slice c * 256 + l of chunk c in wnd[c & 1][l] while chunk c - 1 is
+  // walked (at its first slice): that buffer was last read before the barrier that
+  // ended chunk c - 2.  The window edge is 16 when the slice footprint fits.
+  auto place = [&](int s1) {
+    if (s1 < nsl)
+    {
+      int   ox, oy2;
+      float sx, sy;
+      window(kfirst + s1 * dirSign, ox, oy2, sx, sy);
+      wnd[(s1 / kPlaceChunk) & 1][s1 % kPlaceChunk] =
+        make_int4(ox, oy2, (sx + 5.f <= 16.f && sy + 5.f <= 16.f) ? 16 : kTileWin, 0);
+    }
   };
-  if (tid == 0)
-    place(kfirst, wpar[0]);
+  place(tid);
   __syncthreads();
   for (int s = 0; s <= nsl; ++s)
   {
@@ -689,21 +701,22 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
     int       lim = kTileWin; // window edge used for this slice
     if (s < nsl)
     {
-      wx = wpar[s & 1][0];
-      wy = wpar[s & 1][1];
-      lim = wpar[s & 1][2];
-      if (tid == 0 && s + 1 < nsl)
-        place(k + dirSign, wpar[(s + 1) & 1]);
+      const int4 wv = wnd[(s / kPlaceChunk) & 1][s % kPlaceChunk];
+      wx = __builtin_amdgcn_readfirstlane(wv.x);
+      wy = __builtin_amdgcn_readfirstlane(wv.y);
+      lim = __builtin_amdgcn_readfirstlane(wv.z);
+      if (s % kPlaceChunk == 0)
+        place(s + kPlaceChunk + tid);
       if (s >= r0 && s <= r1)
       {
         unsigned long long * w = win[s & 1];
         float *              g = buf + size_t(k) * oz;
-        int     tx[4], ty[4];
-        float   tw[4];
-        float   sc;
-        const int t = s - r0;
+        const int            t = s - r0;
         if (t == 0 || t > R.nmid)
         {
+          // first or last step: clamped taps, tested one by one (rare)
+          int    tx[4], ty[4];
+          float  tw[4];
           double step, x, y;
           if (t == 0)
           {
@@ -719,46 +732,51 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
             y = wk.PosY();
           }
           BorderTaps2D(R, x, y, tx, ty, tw);
-          sc = float(vn * step);
+          const float sc = float(vn * step);
+          float       v[4];
+          int         outside = 0;
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+          {
+            const int  a = tx[q] - wx, b = ty[q] - wy;
+            const bool in = unsigned(a) < unsigned(lim) && unsigned(b) < unsigned(lim);
+            v[q] = sc * tw[q];
+            atomicAdd(&w[in ? int(__umul24(unsigned(b), unsigned(kTileStride))) + a : kTileCells],
+                      ToFixed(v[q], toFixed));
+            outside |= in ? 0 : 1 << q;
+          }
+          if (outside) // rounding at the window edge: rare
+          {
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+              if (outside & (1 << q))
+                unsafeAtomicAdd(g + tx[q] + ty[q] * oy, v[q]);
+          }
         }
         else
         {
+          // middle step: taps (cx, cy), (cx, cy + 1), (cx + 1, cy), (cx + 1, cy + 1)
           int   cx, cy;
           float lx, ly;
           wk.Stage2D(R, cx, cy, lx, ly);
           const float lxc = 1.f - lx, lyc = 1.f - ly;
-          tx[0] = cx;
-          ty[0] = cy;
-          tw[0] = lxc * lyc;
-          tx[1] = cx;
-          ty[1] = cy + 1;
-          tw[1] = lxc * ly;
-          tx[2] = cx + 1;
-          ty[2] = cy;
-          tw[2] = lx * lyc;
-          tx[3] = cx + 1;
-          ty[3] = cy + 1;
-          tw[3] = lx * ly;
-          sc = float(vn);
-        }
-        float v[4];
-        int   outside = 0;
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-        {
-          const int  a = tx[q] - wx, b = ty[q] - wy;
-          const bool in = unsigned(a) < unsigned(lim) && unsigned(b) < unsigned(lim);
-          v[q] = sc * tw[q];
-          atomicAdd(&w[in ? int(__umul24(unsigned(b), unsigned(kTileStride))) + a : kTileWin * kTileStride],
-                    ToFixed(v[q], toFixed));
-          outside |= in ? 0 : 1 << q;
-        }
-        if (outside) // rounding at the window edge: rare
-        {
-#pragma unroll
-          for (int q = 0; q < 4; q++)
-            if (outside & (1 << q))
-              unsafeAtomicAdd(g + tx[q] + ty[q] * oy, v[q]);
+          const float sc = float(vn);
+          const float v0 = sc * (lxc * lyc), v1 = sc * (lxc * ly), v2 = sc * (lx * lyc), v3 = sc * (lx * ly);
+          const int   a = cx - wx, b = cy - wy;
+          const bool  in = unsigned(a) < unsigned(lim - 1) && unsigned(b) < unsigned(lim - 1);
+          unsigned long long * c = w + (in ? int(__umul24(unsigned(b), unsigned(kTileStride))) + a : kTileCells);
+          atomicAdd(c, ToFixed(v0, toFixed));
+          atomicAdd(c + kTileStride, ToFixed(v1, toFixed));
+          atomicAdd(c + 1, ToFixed(v2, toFixed));
+          atomicAdd(c + kTileStride + 1, ToFixed(v3, toFixed));
+          if (!in) // rounding at the window edge: rare
+          {
+            float * gc = g + cx + cy * oy;
+            unsafeAtomicAdd(gc, v0);
+            unsafeAtomicAdd(gc + oy, v1);
+            unsafeAtomicAdd(gc + 1, v2);
+            unsafeAtomicAdd(gc + oy + 1, v3);
+          }
         }
       }
     }
@@ -768,7 +786,9 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
       const int k1 = k - dirSign;
       unsigned long long * w = win[(s - 1) & 1];
       float *              g = buf + size_t(k1) * oz;
-      // a 16 x 16 window is one cell per lane; a 32 x 32 one four
+      // a 16 x 16 window is one cell per lane; a 32 x 32 one four.  Only taps make a
+      // cell non-zero, and taps are voxels of the slice: the range test is a guard.
+      // Offsets fit 32 bits (volumes below 2^31 voxels).
       const int shift = plim == 16 ? 4 : 5;
       for (int c = tid; c < plim * plim; c += 256)
       {
@@ -776,13 +796,14 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
         const int                cell = int(__umul24(unsigned(j), unsigned(kTileStride))) + i;
         const unsigned long long q = w[cell];
         w[cell] = 0ull;
-        const int X = pwx + i, Y = pwy + j;
-        if (q != 0ull && X >= 0 && X < NX && Y >= 0 && Y < NY)
+        const unsigned X = unsigned(pwx + i), Y = unsigned(pwy + j);
+        if (q != 0ull && X < unsigned(NX) && Y < unsigned(NY))
         {
+          const unsigned off = X + Y * unsigned(oy);
           // hi * 2^32 + lo in FP32 (one more rounding than an exact conversion, far
           // below the FP32 atomic's own): no FP64 on the flush
-          const float hi = float(int(q >> 32)), lo = float(unsigned(q));
-          unsafeAtomicAdd(g + X + Y * oy, fmaf(hi, fromFixed32 * 4294967296.f, lo * fromFixed32));
+          const float    hi = float(int(q >> 32)), lo = float(unsigned(q));
+          unsafeAtomicAdd(g + off, fmaf(hi, fromFixed32 * 4294967296.f, lo * fromFixed32));
         }
       }
     }
