@@ -789,10 +789,7 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
       // a 16 x 16 window is one cell per lane; a 32 x 32 one four.  Only taps make a
       // cell non-zero, and taps are voxels of the slice: the range test is a guard.
       // Offsets fit 32 bits (volumes below 2^31 voxels).
-      const int shift = plim == 16 ? 4 : 5;
-      for (int c = tid; c < plim * plim; c += 256)
-      {
-        const int                i = c & (plim - 1), j = c >> shift;
+      auto flushCell = [&](int i, int j) {
         const int                cell = int(__umul24(unsigned(j), unsigned(kTileStride))) + i;
         const unsigned long long q = w[cell];
         w[cell] = 0ull;
@@ -805,6 +802,14 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
           const float    hi = float(int(q >> 32)), lo = float(unsigned(q));
           unsafeAtomicAdd(g + off, fmaf(hi, fromFixed32 * 4294967296.f, lo * fromFixed32));
         }
+      };
+      if (plim == 16)
+        flushCell(tid & 15, tid >> 4);
+      else
+      {
+#pragma unroll
+        for (int c = tid; c < kTileWin * kTileWin; c += 256)
+          flushCell(c & (kTileWin - 1), c / kTileWin);
       }
     }
     pwx = wx;
