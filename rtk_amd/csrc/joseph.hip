@@ -675,6 +675,7 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
   int          emax;
   frexp(double(maxc), &emax);
   const double toFixed = ldexp(1., 50 - emax);
+  const double vnFixed = vn * toFixed; // |w * vnFixed| < 2^50 for weights w <= 1
   const float  fromFixed32 = ldexpf(1.f, emax - 50);
   Walker       wk;
   int          pwx = 0, pwy = 0; // window origin of the previous slice
@@ -759,23 +760,25 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
           int   cx, cy;
           float lx, ly;
           wk.Stage2D(R, cx, cy, lx, ly);
+          // the weights in FP32, their product with value * |stepMM| rounded once, to
+          // the fixed-point grid (the per-ray path rounds it to FP32 first)
           const float lxc = 1.f - lx, lyc = 1.f - ly;
-          const float sc = float(vn);
-          const float v0 = sc * (lxc * lyc), v1 = sc * (lxc * ly), v2 = sc * (lx * lyc), v3 = sc * (lx * ly);
+          const float w0 = lxc * lyc, w1 = lxc * ly, w2 = lx * lyc, w3 = lx * ly;
           const int   a = cx - wx, b = cy - wy;
           const bool  in = unsigned(a) < unsigned(lim - 1) && unsigned(b) < unsigned(lim - 1);
           unsigned long long * c = w + (in ? int(__umul24(unsigned(b), unsigned(kTileStride))) + a : kTileCells);
-          atomicAdd(c, ToFixed(v0, toFixed));
-          atomicAdd(c + kTileStride, ToFixed(v1, toFixed));
-          atomicAdd(c + 1, ToFixed(v2, toFixed));
-          atomicAdd(c + kTileStride + 1, ToFixed(v3, toFixed));
+          atomicAdd(c, ToFixed(w0, vnFixed));
+          atomicAdd(c + kTileStride, ToFixed(w1, vnFixed));
+          atomicAdd(c + 1, ToFixed(w2, vnFixed));
+          atomicAdd(c + kTileStride + 1, ToFixed(w3, vnFixed));
           if (!in) // rounding at the window edge: rare
           {
-            float * gc = g + cx + cy * oy;
-            unsafeAtomicAdd(gc, v0);
-            unsafeAtomicAdd(gc + oy, v1);
-            unsafeAtomicAdd(gc + 1, v2);
-            unsafeAtomicAdd(gc + oy + 1, v3);
+            const float sc = float(vn);
+            float *     gc = g + cx + cy * oy;
+            unsafeAtomicAdd(gc, sc * w0);
+            unsafeAtomicAdd(gc + oy, sc * w1);
+            unsafeAtomicAdd(gc + 1, sc * w2);
+            unsafeAtomicAdd(gc + oy + 1, sc * w3);
           }
         }
       }
