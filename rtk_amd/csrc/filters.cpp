@@ -1031,9 +1031,19 @@ rtkhip_cg_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const rtkhi
     bpw(dev_proj, c->vB.get());
     double * st = c->state.get();
     Throw(rtkhip_dot(nvox, c->vB.get(), c->vB.get(), st + 4, stream));
-    applyA(dev_x, c->vAP.get());
-    Throw(rtkhip_cg_init(nvox, c->vB.get(), c->vAP.get(), c->vR.get(), c->vP.get(), st, stream));
+    // R = B - A X0.  A cold start (X0 = 0, so that x.x = 0 exactly) skips the
+    // projector pair: A X0 is X0 itself there
+    Throw(rtkhip_dot(nvox, dev_x, dev_x, st + 5, stream));
     double h[8];
+    RTKHIP_CHECK(hipMemcpyAsync(h + 5, st + 5, sizeof(double), hipMemcpyDeviceToHost, s));
+    RTKHIP_CHECK(hipStreamSynchronize(s));
+    const float * ax = dev_x;
+    if (h[5] != 0.)
+    {
+      applyA(dev_x, c->vAP.get());
+      ax = c->vAP.get();
+    }
+    Throw(rtkhip_cg_init(nvox, c->vB.get(), ax, c->vR.get(), c->vP.get(), st, stream));
     RTKHIP_CHECK(hipMemcpyAsync(h, st, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
     RTKHIP_CHECK(hipStreamSynchronize(s));
     const double bnorm = std::sqrt(h[4]);
