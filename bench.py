@@ -67,13 +67,22 @@ def cpu_baseline(np, rows, proj, vol, views):
     st = rtkref.image((NU, NU, views), proj.origin, proj.spacing)
     ov = rtkref.image((NX,) * 3, vol.origin, vol.spacing)
     p = proj.tensor[:views].cpu().numpy()
-    v = np.zeros((NX,) * 3, np.float32)
-    t0 = time.perf_counter()
-    rtkref.fdk(rows[:views], ov, v, st, p, threads=threads)
-    dt = time.perf_counter() - t0
+    times = []
+    for _ in range(3):  # median of three runs, warm caches
+        v = np.zeros((NX,) * 3, np.float32)
+        t0 = time.perf_counter()
+        rtkref.fdk(rows[:views], ov, v, st, p, threads=threads)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next(line.split(":", 1)[1].strip() for line in f if line.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     return {"value": round(NX ** 3 * views / dt / 1e9, 4), "unit": "GUPS", "cores": threads, "kind": "port",
             "sample": f"FDK (weight+ramp+BP, float) of the full 512^3 volume from the first {views} of the 720 "
-                      f"views, {dt:.1f} s on {threads} threads"}
+                      f"views, median of 3 runs {dt:.2f} s on {threads} threads of {model}"}
 
 
 def projector_pair_measurement(rtk, np, torch, g, q, proj, vol, reps=2):

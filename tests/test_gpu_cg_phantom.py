@@ -81,3 +81,30 @@ def test_phantom_generators(oracle, hip):
     ov = oracle.image((40, 36, 30), (-60., -52., -58.), (3., 3., 4.))
     vref = oracle.draw_quadrics(ov, np.zeros((30, 36, 40), np.float32), q)
     assert np.mean(vol.numpy() != vref) < 1e-4
+
+
+@pytest.mark.parametrize("joseph_bp", [False, True])
+def test_cg_warm_start_matches_double_oracle(oracle, hip, joseph_bp):
+    """Non-zero start: R0 = B - A X0 takes the projector pair (a cold start skips it,
+    A 0 = 0); the iterate and the residual history follow the double oracle from the
+    same X0, with either backprojector."""
+    rows, q, g, proj = _problem(hip, n=60)
+    ov = oracle.image((64,) * 3, (-127.,) * 3, (4.,) * 3)
+    rng = np.random.default_rng(31)
+    x0 = (0.5 * oracle.draw_quadrics(ov, np.zeros((64,) * 3, np.float32), q)
+          + 0.05 * rng.normal(size=(64,) * 3)).astype(np.float32)
+    f = hip.HipConjugateGradientConeBeamReconstructionFilter()
+    f.SetInput(0, to_device(hip, x0, (-127.,) * 3, (4.,) * 3))
+    f.SetInput(1, proj)
+    f.SetGeometry(g)
+    if joseph_bp:
+        f.SetBackProjectionFilter(f.BP_JOSEPH)
+    f.SetNumberOfIterations(3)
+    f.Update()
+    os_ = oracle.image((64, 64, 60), (-255.,) * 3, (8.,) * 3)
+    xd, rd = oracle.conjugate_gradient(rows, ov, x0.astype(np.float64), os_, proj.numpy().astype(np.float64), 3,
+                                       joseph_bp=joseph_bp)
+    assert rel_l2(f.GetOutput().numpy(), xd) <= 1e-4
+    res = f.GetResiduals()
+    # the HIP residuals are relative to ||B||; the oracle's are squared norms of R_k
+    np.testing.assert_allclose(np.asarray(res) / res[0], np.sqrt(rd / rd[0]), rtol=1e-3)
