@@ -254,6 +254,8 @@ def warm_start_measurement(rtk, np, torch, seeds, tol):
             "median_seconds": {k: round(v, 3) for k, v in med.items()},
             "median_iterations": {k: float(np.median([r[k]["iterations"] for r in rows]))
                                   for k in ("zero", "fdk", "cnn")},
+            "median_rmse_vs_phantom": {k: float(np.median([r[k]["rmse_vs_phantom"] for r in rows]))
+                                       for k in ("zero", "fdk", "cnn")},
             "speedup_cnn_vs_zero": round(med["zero"] / med["cnn"], 3),
             "speedup_fdk_vs_zero": round(med["zero"] / med["fdk"], 3),
             "per_seed": rows}

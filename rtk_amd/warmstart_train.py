@@ -114,6 +114,9 @@ def evaluate_seed(net, g, seed, tol=1e-3, max_iter=200):
     phantom seed; the start's own cost (FDK, CNN inference) is part of its time."""
     vol = volume_source()
     proj = noisy_projections(g, seed)
+    # the voxelised phantom: every end point's distance to the truth, to show that
+    # the faster start does not buy its speed with a worse volume
+    truth = rtk.draw_quadrics(vol, rtk.phantoms.random_ellipsoids(seed, n=20, half_fov=NX / 2.)).tensor
     torch.cuda.synchronize()
     out = {"seed": int(seed)}
     vols = {}
@@ -133,7 +136,8 @@ def evaluate_seed(net, g, seed, tol=1e-3, max_iter=200):
         t2 = time.perf_counter()
         out[name] = {"seconds": round(t2 - t0, 4), "start_seconds": round(t1 - t0, 4), "iterations": int(it),
                      "start_rel_residual": float(res[0]), "final_rel_residual": float(res[-1]),
-                     "reached": bool(res[-1] <= tol)}
+                     "reached": bool(res[-1] <= tol),
+                     "rmse_vs_phantom": float(torch.sqrt(torch.mean((x.tensor - truth) ** 2)).item())}
         vols[name] = x.tensor
     # the end points all sit at the same residual; their spread is the solution's own slack
     for name in ("fdk", "cnn"):
@@ -230,6 +234,7 @@ def main():
         for name in ("zero", "fdk", "cnn"):
             rep[f"median_{name}_seconds"] = float(np.median([r[name]["seconds"] for r in rows]))
             rep[f"median_{name}_iterations"] = float(np.median([r[name]["iterations"] for r in rows]))
+            rep[f"median_{name}_rmse_vs_phantom"] = float(np.median([r[name]["rmse_vs_phantom"] for r in rows]))
         rep["speedup_cnn_vs_zero"] = rep["median_zero_seconds"] / rep["median_cnn_seconds"]
         rep["speedup_fdk_vs_zero"] = rep["median_zero_seconds"] / rep["median_fdk_seconds"]
         print(json.dumps(rep), flush=True)
