@@ -2,12 +2,13 @@
 
 Sparse-view configuration (256^3 @1 mm, 90 views of 512^2 @0.8 mm, SID 1000 mm,
 SDD 1536 mm, Poisson noise I0 = 1e4): for each seed, the random-ellipsoid phantom is
-projected analytically, noise is applied, and the pair (FDK volume, volume after 50
-CG iterations from zero) is produced by this package's own HIP filters.  Fifty
-iterations take the relative residual to about 6e-4, below the 1e-3 tolerance the
-start is judged at; unregularised CG keeps fitting noise after that, and a network
-asked for the 100-iteration volume must guess noise it cannot see (measured: a
-smaller gain).  `--cg-iter 100` trains toward the 100-iteration volume.  CG uses the
+projected analytically, noise is applied, and the pair (FDK volume, volume after 40
+CG iterations from zero) is produced by this package's own HIP filters.  Forty
+iterations take the relative residual to about 7e-4, below the 1e-3 tolerance the
+start is judged at.  Unregularised CG keeps fitting noise after that: a network asked
+for a later iterate starts CG from a noisier volume (its end point drifts from the
+phantom), and one asked for the 100-iteration volume must guess noise it cannot see
+(a smaller gain).  `--cg-iter` selects the iterate.  CG uses the
 Joseph forward projector with its matched Joseph backprojector, so the normal
 operator BP W FP is symmetric (RTK's CG with the voxel-based backprojector is not,
 and stalls at this view count).  The CNN is trained on random crops with Adam and
@@ -79,7 +80,7 @@ def cg(g, proj, x0, niter, tol=0.):
     return f.GetOutput(), f.GetResiduals(), f.GetNumberOfIterationsDone()
 
 
-def make_pair(g, seed, cg_iter=50):
+def make_pair(g, seed, cg_iter=40):
     vol = volume_source()
     proj = noisy_projections(g, seed)
     x_fdk = fdk(g, proj, vol)
@@ -209,7 +210,7 @@ def main():
     ap.add_argument("--crop", type=int, default=64)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--cg-iter", type=int, default=50)
+    ap.add_argument("--cg-iter", type=int, default=40)
     ap.add_argument("--out", default="gpurun_out/warmstart_c4.npz")
     ap.add_argument("--report", default="gpurun_out/warmstart_c4.json")
     ap.add_argument("--eval-only", action="store_true",
@@ -261,6 +262,7 @@ def main():
     for name in ("zero", "fdk", "cnn"):
         rep[f"median_{name}_seconds"] = float(np.median([r[name]["seconds"] for r in rows]))
         rep[f"median_{name}_iterations"] = float(np.median([r[name]["iterations"] for r in rows]))
+        rep[f"median_{name}_rmse_vs_phantom"] = float(np.median([r[name]["rmse_vs_phantom"] for r in rows]))
     rep["speedup_cnn_vs_zero"] = rep["median_zero_seconds"] / rep["median_cnn_seconds"]
     with open(args.report, "w") as f:
         json.dump(rep, f, indent=1)
