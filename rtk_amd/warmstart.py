@@ -26,6 +26,14 @@ def default_weights_path():
     return os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "warmstart_c4.npz")
 
 
+def scale_weights_path():
+    """The network the at-scale workload (512^3 @0.5 mm, 180 views of 1024^2 @0.4 mm,
+    I0 = 5e3) uses by default, shipped with the package: trained on the sparse-view
+    workload toward the 50-iteration CG volume (python -m rtk_amd.warmstart_train
+    --cg-iter 50), the one that transferred best to the at-scale workload."""
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "warmstart_c5.npz")
+
+
 def num_params():
     return lib().rtkhip_cnn_num_params()
 

@@ -18,9 +18,11 @@ included in its time.
 
 Usage: python -m rtk_amd.warmstart_train --train 320 --steps 8000 --eval 16 --out w.npz
        python -m rtk_amd.warmstart_train --scale-eval 1   (512^3 / 180 views, tiled CNN)
+       python -m rtk_amd.warmstart_train --scale-train 32 --steps 4000 --scale-eval 1 --out w5.npz
 """
 import argparse
 import json
+import os
 import sys
 import time
 
@@ -92,7 +94,7 @@ def random_crops(pairs, crop, batch, rng):
     xs, ts = [], []
     for _ in range(batch):
         x, t = pairs[rng.integers(len(pairs))][:2]
-        z, y, w = (int(rng.integers(0, NX - crop + 1)) for _ in range(3))
+        z, y, w = (int(rng.integers(0, n - crop + 1)) for n in x.shape)
         xs.append(x[z:z + crop, y:y + crop, w:w + crop])
         ts.append(t[z:z + crop, y:y + crop, w:w + crop])
     return torch.stack(xs).contiguous(), torch.stack(ts).contiguous()
@@ -158,12 +160,10 @@ def evaluate(net, g, seeds, tol=1e-3, max_iter=200):
     return rows
 
 
-def evaluate_at_scale(net, seed, tol=1e-3, max_iter=300, tile=128, halo=8):
-    """The learned start at scale: 512^3 @0.5 mm from 180 views of 1024^2 @0.4 mm
-    (SID 1000 mm, SDD 1536 mm), 30 random ellipsoids, Poisson noise I0 = 5e3.  The
-    network runs as overlapping tile^3 blocks with `halo` voxels of context, then CG
-    (Joseph pair, FP64 scalars) runs to relative residual `tol` from zero, FDK and
-    CNN(FDK).  Returns the same fields as evaluate_seed."""
+def scale_problem(seed):
+    """The at-scale workload: 512^3 @0.5 mm from 180 views of 1024^2 @0.4 mm (SID
+    1000 mm, SDD 1536 mm), 30 random ellipsoids, Poisson noise I0 = 5e3.  Returns
+    (geometry, noisy projections, volume source, quadrics)."""
     views, nu, nx = 180, 1024, 512
     g = geometry(views)
     vol = rtk.ConstantImageSource((nx, nx, nx), (0.5, 0.5, 0.5), (-(nx - 1) * 0.5 / 2,) * 3)
@@ -175,6 +175,24 @@ def evaluate_at_scale(net, seed, tol=1e-3, max_iter=300, tile=128, halo=8):
     counts = torch.poisson(5e3 * torch.exp(-clean.tensor.double()), generator=gen)
     proj = clean.like((-torch.log(torch.clamp(counts, min=1.) / 5e3)).float().contiguous())
     del clean, counts
+    return g, proj, vol, q
+
+
+def make_scale_pair(seed, cg_iter=40, tile=128, halo=8):
+    """(FDK, cg_iter-iteration CG volume) of one at-scale phantom."""
+    g, proj, vol, _ = scale_problem(seed)
+    x_fdk = fdk(g, proj, vol)
+    x_cg, res, _ = cg(g, proj, vol, cg_iter)
+    return x_fdk.tensor, x_cg.tensor, float(res[-1])
+
+
+def evaluate_at_scale(net, seed, tol=1e-3, max_iter=300, tile=128, halo=8):
+    """The learned start at scale (scale_problem).  The network runs as overlapping
+    tile^3 blocks with `halo` voxels of context, then CG (Joseph pair, FP64 scalars)
+    runs to relative residual `tol` from zero, FDK and CNN(FDK).  Returns the same
+    fields as evaluate_seed."""
+    g, proj, vol, q = scale_problem(seed)
+    truth = rtk.draw_quadrics(vol, q).tensor
     out = {"seed": int(seed)}
     vols = {}
     for name in ("zero", "fdk", "cnn"):
@@ -193,7 +211,8 @@ def evaluate_at_scale(net, seed, tol=1e-3, max_iter=300, tile=128, halo=8):
         t2 = time.perf_counter()
         out[name] = {"seconds": round(t2 - t0, 3), "start_seconds": round(t1 - t0, 4), "iterations": int(it),
                      "start_rel_residual": float(res[0]), "final_rel_residual": float(res[-1]),
-                     "reached": bool(res[-1] <= tol)}
+                     "reached": bool(res[-1] <= tol),
+                     "rmse_vs_phantom": float(torch.sqrt(torch.mean((x.tensor - truth) ** 2)).item())}
         vols[name] = x.tensor
         log(f"scale seed {seed} {name}: {out[name]}")
     for name in ("fdk", "cnn"):
@@ -216,12 +235,34 @@ def main():
     ap.add_argument("--eval-only", action="store_true",
                     help="evaluate the shipped network on --eval held-out seeds, without training")
     ap.add_argument("--scale-eval", type=int, default=0,
-                    help="only evaluate the shipped network at 512^3 / 180 views on this many held-out seeds")
+                    help="evaluate at 512^3 / 180 views on this many held-out seeds (after --scale-train if given)")
+    ap.add_argument("--scale-train", type=int, default=0,
+                    help="train a network on this many 512^3 / 180-view pairs (toward the --cg-iter volume)")
+    ap.add_argument("--weights", default=None, help="network for --eval-only / --scale-eval (default: shipped)")
     args = ap.parse_args()
     rtk.lib()
-    if args.scale_eval:
-        from rtk_amd.warmstart import default_weights_path
-        net = WarmStartCNN.load(default_weights_path())
+    if args.scale_train or args.scale_eval:
+        from rtk_amd.warmstart import default_weights_path, scale_weights_path
+        if args.scale_train:
+            t0 = time.perf_counter()
+            pairs = []
+            for i in range(args.scale_train):
+                x, t, r = make_scale_pair(TRAIN_SEED0 + 100 + i, args.cg_iter)
+                pairs.append((x, t))
+                log(f"scale pairs {i + 1}/{args.scale_train} ({time.perf_counter() - t0:.0f} s), last CG residual {r:.3g}")
+            scale = float(1.0 / torch.stack([p[1].abs().mean() for p in pairs]).mean().item())
+            net = WarmStartCNN(scale=scale, seed=0)
+            t1 = time.perf_counter()
+            losses = train(net, pairs, args.steps, args.crop, args.batch, args.lr)
+            torch.cuda.synchronize()
+            log(f"scale training: {args.scale_train} pairs {t1 - t0:.0f} s, {args.steps} steps "
+                f"{time.perf_counter() - t1:.0f} s, loss {np.mean(losses[:20]):.3g} -> {np.mean(losses[-20:]):.3g}")
+            del pairs
+            net.save(args.out)
+        else:
+            path = args.weights or (scale_weights_path() if os.path.exists(scale_weights_path())
+                                    else default_weights_path())
+            net = WarmStartCNN.load(path)
         rows = [evaluate_at_scale(net, EVAL_SEED0 + 100 + i) for i in range(args.scale_eval)]
         print(json.dumps({"workload": "CG 512^3 @0.5 mm <- 180 x 1024^2 @0.4 mm, 30 ellipsoids, I0=5e3",
                           "eval": rows}), flush=True)
@@ -229,7 +270,7 @@ def main():
     g = geometry()
     if args.eval_only:
         from rtk_amd.warmstart import default_weights_path
-        net = WarmStartCNN.load(default_weights_path())
+        net = WarmStartCNN.load(args.weights or default_weights_path())
         rows = evaluate(net, g, [EVAL_SEED0 + i for i in range(args.eval)])
         rep = {"workload": "CG 256^3 @1 mm <- 90 x 512^2 @0.8 mm, Poisson I0=1e4, held-out seeds", "eval": rows}
         for name in ("zero", "fdk", "cnn"):
