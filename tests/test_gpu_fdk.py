@@ -154,3 +154,33 @@ def test_fdk_subsets_hann_and_half_stacks(oracle, hip):
     # 16-bit filtered stacks: stated error against the FP64 oracle
     assert rel_l2(out["f16"], ref_d) <= 1e-3
     assert rel_l2(out["bf16"], ref_d) <= 1e-2
+
+
+def test_fdk_noise_hann_reference_thresholds(oracle, hip):
+    """test/rtkrampfiltertest.cxx:86-113: Shepp-Logan (scale 128) from 180 views of
+    128^2 at 4 mm (SID 600, SDD 1200) with additive Gaussian noise of standard
+    deviation 1, FDK with a Hann window cut at 0.8: error per voxel < 0.72 and
+    PSNR > 22.4 dB (reference value 2) against the drawn phantom over the whole
+    128^3 @2 mm volume; the HIP volume also matches the float oracle on the same
+    noisy stack (1e-4)."""
+    from rtk_amd import phantoms
+    n = 180
+    rows = np.array([[600., 1200., i * 360. / n, 0, 0, 0, 0, 0, 0] for i in range(n)])
+    q = phantoms.shepp_logan(128.)
+    g = hip_geometry(hip, rows)
+    clean = hip.project_quadrics(g, hip.ConstantImageSource((128, 128, n), (4., 4., 4.), (-254.,) * 3, 0.), q)
+    noisy = clean.numpy() + np.random.default_rng(12).normal(0., 1., size=(n, 128, 128)).astype(np.float32)
+    f = hip.HipFDKConeBeamReconstructionFilter()
+    f.SetInput(0, hip.ConstantImageSource((128, 128, 128), (2., 2., 2.), (-127.,) * 3, 0.))
+    f.SetInput(1, to_device(hip, noisy, (-254.,) * 3, (4.,) * 3))
+    f.SetGeometry(g)
+    f.GetRampFilter().SetHannCutFrequency(0.8)
+    f.Update()
+    rec = f.GetOutput().numpy()
+    ov = oracle.image((128,) * 3, (-127.,) * 3, (2.,) * 3)
+    os_ = oracle.image((128, 128, n), (-254.,) * 3, (4.,) * 3)
+    ref = oracle.fdk(rows, ov, np.zeros((128,) * 3, np.float32), os_, noisy, hann=0.8)
+    assert rel_l2(rec, ref) <= 1e-4
+    phantom = oracle.draw_quadrics(ov, np.zeros((128,) * 3, np.float32), q)
+    err, psnr, ok = check_image_quality(rec, phantom, 0.72, 22.4, 2.)
+    assert ok, (err, psnr)
