@@ -7,7 +7,9 @@ generators of librtkhip (and the oracle).
   include/rtkSheppLoganPhantomFilter.hxx:29) as ProjectGeometricPhantom does
   (rotate -> translate -> rescale, include/rtkProjectGeometricPhantomImageFilter.hxx:71-74).
 * ``random_ellipsoids`` is the seeded random-ellipsoid phantom of the benchmark
-  configurations (its parameters are this project's definition).
+  workloads: a 0.02 mm^-1 base sphere plus ellipsoids with random centres, semi-axes,
+  orientations and density offsets (the distributions are in its docstring; they are
+  this project's definition, RTK has no such generator).
 """
 import numpy as np
 

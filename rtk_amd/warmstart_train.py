@@ -5,7 +5,7 @@ SDD 1536 mm, Poisson noise I0 = 1e4): for each seed, the random-ellipsoid phanto
 projected analytically, noise is applied, and the pair (FDK volume, volume after 40
 CG iterations from zero) is produced by this package's own HIP filters.  Forty
 iterations take the relative residual to about 7e-4, below the 1e-3 tolerance the
-start is judged at.  Unregularised CG keeps fitting noise after that: a network asked
+start is evaluated at.  Unregularised CG keeps fitting noise after that: a network asked
 for a later iterate starts CG from a noisier volume (its end point drifts from the
 phantom), and one asked for the 100-iteration volume must guess noise it cannot see
 (a smaller gain).  `--cg-iter` selects the iterate.  CG uses the
@@ -230,8 +230,8 @@ def main():
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--cg-iter", type=int, default=40)
-    ap.add_argument("--out", default="gpurun_out/warmstart_c4.npz")
-    ap.add_argument("--report", default="gpurun_out/warmstart_c4.json")
+    ap.add_argument("--out", default="warmstart_c4.npz")
+    ap.add_argument("--report", default="warmstart_c4.json")
     ap.add_argument("--eval-only", action="store_true",
                     help="evaluate the shipped network on --eval held-out seeds, without training")
     ap.add_argument("--scale-eval", type=int, default=0,
