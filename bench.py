@@ -326,7 +326,7 @@ def main():
                     help="only the Joseph FP / BP pair at the sparse-view CG shape")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
+        print("bench.py measures one GPU only: rtkhip has no multi-GPU path "
               "(the single-GPU kernels run at the per-GPU share's full shape).", file=sys.stderr)
         sys.exit(2)
 

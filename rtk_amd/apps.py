@@ -62,7 +62,7 @@ def _reject(args, names, defaults=None):
         v = getattr(args, n, None)
         if v is not None and v != defaults.get(n) and not (isinstance(v, list) and all(x == defaults.get(n, 0)
                                                                                        for x in v)):
-            raise SystemExit(f"--{n}: not supported by this build (processing outside the projector path)")
+            raise SystemExit(f"--{n}: not supported by rtkhip (processing outside the projector path)")
 
 
 def read_projections(args, verbose=False):
@@ -79,7 +79,7 @@ def read_projections(args, verbose=False):
         a, o, s = io.read_image(f)
         if not np.issubdtype(a.dtype, np.floating) and not args.nolineint:
             raise SystemExit(f"{f}: integer projections need the scanner's raw-to-line-integral conversion, which "
-                             "this build does not implement; pass --nolineint to use the values as line integrals")
+                             "rtkhip does not implement; pass --nolineint to use the values as line integrals")
         if a.ndim == 2:
             a = a[None]
             o, s = tuple(o) + (0.,), tuple(s) + (1.,)
@@ -140,7 +140,7 @@ def rtkfdk(argv=None):
     _add_output_image(p)
     args = p.parse_args(argv)
     if args.signal or args.dvf:
-        raise SystemExit("--signal/--dvf: motion compensation is not part of this build")
+        raise SystemExit("--signal/--dvf: motion compensation is not part of rtkhip")
     proj = read_projections(args, args.verbose)
     g = io.read_geometry(args.geometry)
     ddf = HipDisplacedDetectorImageFilter()
@@ -196,9 +196,9 @@ def rtkconjugategradient(argv=None):
     p.add_argument("--mask", "-m", help=argparse.SUPPRESS)
     p.add_argument("--nodisplaced", action="store_true", help="Disable the displaced detector filter")
     p.add_argument("--tolerance", type=float, default=0.,
-                   help="Stop once ||b - A x|| / ||b|| <= tolerance (this build's extension; 0 = n iterations)")
+                   help="Stop once ||b - A x|| / ||b|| <= tolerance (an rtkhip extension; 0 = n iterations)")
     p.add_argument("--stackdtype", choices=["f32", "f16", "bf16"], default="f32",
-                   help="Storage of the stacks the backprojector reads (this build's extension)")
+                   help="Storage of the stacks the backprojector reads (an rtkhip extension)")
     pr = p.add_argument_group("Projectors")
     pr.add_argument("--fp", "-f", choices=sorted(_FP), default="Joseph", help="Forward projection method")
     pr.add_argument("--bp", "-b", choices=sorted(_BP), default="VoxelBasedBackProjection",
@@ -207,7 +207,7 @@ def rtkconjugategradient(argv=None):
     _add_output_image(p)
     args = p.parse_args(argv)
     if args.regweights or args.mask or args.gamma or args.tikhonov:
-        raise SystemExit("--regweights/--mask/--gamma/--tikhonov: regularised CG is not part of this build")
+        raise SystemExit("--regweights/--mask/--gamma/--tikhonov: regularised CG is not part of rtkhip")
     proj = read_projections(args, args.verbose)
     g = io.read_geometry(args.geometry)
     if args.input:
