@@ -1,7 +1,7 @@
 """FP64 oracle of the warm-start CNN (test infrastructure only).
 
-The network has no counterpart in the reference; this module is the checker for the
-HIP kernels of rtk_amd/csrc/cnn.hip.  Convolutions are cross-correlations with zero
+The network has no counterpart in the reference; this module is the correctness
+reference for the HIP kernels of rtk_amd/csrc/cnn.hip.  Convolutions are cross-correlations with zero
 padding, tap t = (dz*3 + dy)*3 + dx, offset (dx-1, dy-1, dz-1); arrays are
 channels-last ``[b, z, y, x, c]``.  Pinned by finite differences of its own forward
 (tests/test_oracle_cnn.py) and by torch's float64 CPU conv3d as an independent route.

@@ -4,7 +4,8 @@
 // filters, templated on the pixel type (float = the reference's float path,
 // double = the FP64 oracle "same filters on double pixels").  Only tests/,
 // __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and only
-// as the checker or the timed CPU baseline -- never as the product path.
+// as the correctness reference or the timed CPU baseline -- never as the product
+// path.
 //
 // Every function cites the reference file:line whose behaviour it restates.
 // Coordinates, matrices and interpolation weights are computed in double as RTK

@@ -1,7 +1,8 @@
 """ctypes front end of the CPU oracle ``librtkref.so`` (test infrastructure only).
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline``
-leg may import this module, and only as the checker or the timed CPU baseline.
+leg may import this module, and only as the correctness reference or the timed CPU
+baseline.
 Arrays are numpy, x-fastest (index order ``[z, y, x]`` / ``[view, v, u]``).
 Geometry rows are the nine ``AddProjection`` arguments
 (src/rtkThreeDCircularProjectionGeometry.cxx:55-76).
