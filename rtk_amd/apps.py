@@ -228,6 +228,10 @@ def rtkconjugategradient(argv=None):
         a, o, s = io.read_image(args.weights)
         if a.ndim == 2:
             a = a[None]
+        if tuple(a.shape) != tuple(proj.tensor.shape):
+            # RTK's pipeline fails on the mismatched requested region of the weights input
+            raise SystemExit(f"--weights: image of size {tuple(reversed(a.shape))} does not match the projections' "
+                             f"size {proj.size}")
         f.SetInputWeights(Image(torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda(), proj.origin,
                                 proj.spacing))
     t0 = time.perf_counter()

@@ -163,9 +163,14 @@ def ConstantImageSource(size, spacing=(1., 1., 1.), origin=(0., 0., 0.), constan
     return Image(t, origin, spacing)
 
 
-def _require_cuda(img, what):
-    if not img.tensor.is_cuda or img.tensor.dtype not in (torch.float32, torch.float16, torch.bfloat16):
-        raise ValueError(f"{what}: expected a CUDA float tensor")
+def _require_cuda(img, what, allow16=False):
+    """Every image the library reads or writes is a contiguous CUDA tensor of float32
+    pixels; only the projection stacks the backprojectors read may be float16 or
+    bfloat16 (the C entry points take FP32 buffers everywhere else, so a 16-bit
+    image there would be read and written at twice its size)."""
+    ok = (torch.float32, torch.float16, torch.bfloat16) if allow16 else (torch.float32,)
+    if not isinstance(img, Image) or not img.tensor.is_cuda or img.tensor.dtype not in ok:
+        raise ValueError(f"{what}: expected a CUDA {'float32/float16/bfloat16' if allow16 else 'float32'} image")
 
 
 class _Filter:
@@ -325,7 +330,7 @@ class HipBackProjectionImageFilter(_Filter):
     def Update(self):
         vol, proj = self._inputs[0], self._inputs[1]
         _require_cuda(vol, "BP volume")
-        _require_cuda(proj, "BP projections")
+        _require_cuda(proj, "BP projections", allow16=True)
         dt = {torch.float32: _lib.F32, torch.float16: _lib.F16, torch.bfloat16: _lib.BF16}[proj.tensor.dtype]
         out = self._out_for(vol)
         vd, pd = vol.descriptor(), proj.descriptor()
@@ -341,7 +346,7 @@ class HipJosephBackProjectionImageFilter(_Filter):
     def Update(self):
         vol, proj = self._inputs[0], self._inputs[1]
         _require_cuda(vol, "BP volume")
-        _require_cuda(proj, "BP projections")
+        _require_cuda(proj, "BP projections", allow16=True)
         dt = {torch.float32: _lib.F32, torch.float16: _lib.F16, torch.bfloat16: _lib.BF16}[proj.tensor.dtype]
         out = self._out_for(vol)
         vd, pd = vol.descriptor(), proj.descriptor()
@@ -505,6 +510,9 @@ class HipConjugateGradientConeBeamReconstructionFilter(_Filter):
         self._bp = int(bp)
 
     def SetInputWeights(self, w):
+        """Projection weights W: a float32 CUDA image of exactly the projections' size
+        (checked again at Update, against the projections then set)."""
+        _require_cuda(w, "CG weights")
         self._weights = w
 
     def SetResidualTolerance(self, tol):
@@ -521,6 +529,11 @@ class HipConjugateGradientConeBeamReconstructionFilter(_Filter):
         vol, proj = self._inputs[0], self._inputs[1]
         _require_cuda(vol, "CG volume")
         _require_cuda(proj, "CG projections")
+        if self._weights is not None:
+            _require_cuda(self._weights, "CG weights")
+            if self._weights.size != proj.size:
+                raise ValueError(f"CG weights: size {self._weights.size} differs from the projections' "
+                                 f"{proj.size}")
         x = vol.tensor.clone()
         res = np.zeros(self._niter + 1)
         it = ctypes.c_int()
@@ -547,6 +560,7 @@ class HipConjugateGradientConeBeamReconstructionFilter(_Filter):
 
 def draw_quadrics(img, quadrics, inplace=False):
     """DrawGeometricPhantom-style voxelisation of quadric shapes (11 numbers each)."""
+    _require_cuda(img, "phantom volume")
     q = np.ascontiguousarray(np.asarray(quadrics, dtype=np.float64).reshape(-1, 11))
     out = img.tensor if inplace else torch.empty_like(img.tensor)
     d = img.descriptor()
@@ -557,6 +571,7 @@ def draw_quadrics(img, quadrics, inplace=False):
 
 def project_quadrics(geometry, proj, quadrics, inplace=False):
     """ProjectGeometricPhantom-style analytic line integrals (FP64 chords)."""
+    _require_cuda(proj, "phantom projections")
     q = np.ascontiguousarray(np.asarray(quadrics, dtype=np.float64).reshape(-1, 11))
     out = proj.tensor if inplace else torch.empty_like(proj.tensor)
     d = proj.descriptor()

@@ -95,8 +95,11 @@ def random_ellipsoids(seed, n=20, half_fov=128.0, base=0.02, base_radius=0.9, ce
     return np.stack(rows)
 
 
-def poisson_line_integrals(p, i0, seed):
-    """Noisy line integrals: counts ~ Poisson(I0 exp(-p)), p_hat = -ln(max(c, 1) / I0)."""
+def poisson_line_integrals(p, i0, seed, return_counts=False):
+    """Noisy line integrals: counts ~ Poisson(I0 exp(-p)), p_hat = -ln(max(c, 1) / I0)
+    (numpy's generator, so the same float32 p and seed give the same draw on any host).
+    return_counts: also return the counts."""
     rng = np.random.default_rng(seed)
     c = rng.poisson(i0 * np.exp(-np.asarray(p, dtype=np.float64)))
-    return (-np.log(np.maximum(c, 1) / i0)).astype(np.float32)
+    out = (-np.log(np.maximum(c, 1) / i0)).astype(np.float32)
+    return (out, c) if return_counts else out

@@ -1,0 +1,138 @@
+"""Plain CG at its full benchmark size against the double-precision oracle.
+
+Workload: 256^3 @1 mm from 360 views of 512^2 @0.8 mm (SID 1000 mm, SDD 1536 mm),
+analytic projections of the seeded 20-ellipsoid phantom with Poisson noise at
+I0 = 1e4 drawn by numpy (the same draw tests/golden/make_cg_c3.py made), 30
+iterations from zero with RTK's default pair (Joseph forward, voxel-based back
+projection).  tests/golden/cg_c3.npz holds the oracle's residual history, central
+planes and 8^3 block means of X_30, computed in double.
+
+Checks:
+  * FP32 stacks: the residual history ||R_k|| / ||B|| follows the oracle's, and X_30's
+    planes and block means match it;
+  * ||B - A X_30|| / ||B|| recomputed directly in FP64 on the device (a fresh FP and BP
+    of X_30, not the CG recursion) agrees with the recursion's last residual;
+  * the first step is X_1 = alpha_0 B with alpha_0 = B.B / B.AB, in FP64;
+  * FP16 and BF16 stacks: X_30's error against the oracle, within the bars stated at
+    test_c3_16bit_stacks_vs_fp64_oracle.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import hip_geometry, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURE = os.path.join(HERE, "golden", "cg_c3.npz")
+NX, NU, NVIEWS = 256, 512, 360
+ROWS = np.array([[1000., 1536., i * 360. / NVIEWS, 0., 0., 0., 0., 0., 0.] for i in range(NVIEWS)])
+
+
+@pytest.fixture(scope="module")
+def c3(hip):
+    if not os.path.exists(FIXTURE):
+        pytest.skip("tests/golden/cg_c3.npz has not been generated")
+    fx = dict(np.load(FIXTURE))
+    from rtk_amd import phantoms
+    g = hip_geometry(hip, ROWS)
+    q = phantoms.random_ellipsoids(int(fx["phantom_seed"]), n=20, half_fov=NX / 2.)
+    det0 = -(NU - 1) * 0.8 / 2
+    clean = hip.project_quadrics(g, hip.ConstantImageSource((NU, NU, NVIEWS), (0.8, 0.8, 1.), (det0, det0, 0.)), q)
+    p, counts = phantoms.poisson_line_integrals(clean.numpy(), float(fx["i0"]), int(fx["noise_seed"]),
+                                                return_counts=True)
+    # the HIP and oracle phantom projectors agree to FP64 rounding, so the float32 line
+    # integrals, and with them the Poisson draw, are the oracle's
+    assert int(counts.sum()) == int(fx["counts_sum"]), (int(counts.sum()), int(fx["counts_sum"]))
+    proj = clean.like(torch.from_numpy(p).cuda())
+    vol = hip.ConstantImageSource((NX,) * 3, (1., 1., 1.), (-(NX - 1) / 2.,) * 3)
+    return g, proj, vol, fx
+
+
+def _cg(hip, c3, niter, dtype="f32"):
+    g, proj, vol, _ = c3
+    f = hip.HipConjugateGradientConeBeamReconstructionFilter()
+    f.SetInput(0, vol)
+    f.SetInput(1, proj)
+    f.SetGeometry(g)
+    f.SetStackDtype(dtype)
+    f.SetNumberOfIterations(niter)
+    f.Update()
+    return f.GetOutput(), f.GetResiduals()
+
+
+def _vs_oracle(x, fx):
+    a = x.tensor.double()
+    bm = a.reshape(32, 8, 32, 8, 32, 8).mean(dim=(1, 3, 5)).cpu().numpy()
+    return {"slice_y": rel_l2(a[:, NX // 2, :].cpu().numpy(), fx["slice_y"]),
+            "slice_z": rel_l2(a[NX // 2].cpu().numpy(), fx["slice_z"]),
+            "block_means": rel_l2(bm, fx["block_means"])}
+
+
+def _apply_a(hip, g, proj, x):
+    """A x = BP(FP(x)) with the filters the CG operator is made of (W = 1)."""
+    fp = hip.HipForwardProjectionImageFilter()
+    fp.SetInput(0, proj.like(torch.zeros_like(proj.tensor)))
+    fp.SetInput(1, x)
+    fp.SetGeometry(g)
+    fp.Update()
+    return _bp(hip, g, fp.GetOutput(), x)
+
+
+def _bp(hip, g, p, like):
+    bp = hip.HipBackProjectionImageFilter()
+    bp.SetInput(0, like.like(torch.zeros_like(like.tensor)))
+    bp.SetInput(1, p)
+    bp.SetGeometry(g)
+    bp.Update()
+    return bp.GetOutput().tensor
+
+
+def test_c3_fp32_residual_history_and_volume_vs_oracle(hip, c3):
+    g, proj, vol, fx = c3
+    x, res = _cg(hip, c3, 30)
+    dev = np.abs(res / fx["residual"] - 1.)
+    err = _vs_oracle(x, fx)
+    print(f"C3 FP32: residual history max rel dev {dev.max():.3g} (last {res[-1]:.6g} vs {fx['residual'][-1]:.6g}); "
+          f"X_30 vs oracle {err}")
+    # FP32 projector rounding (~1e-6 relative per operator application) grows through
+    # the 30 recursions of an unsymmetric operator; the bars leave a 3x margin
+    assert dev.max() <= 1e-4
+    assert max(err.values()) <= 1e-4
+    # the recursion's residual against a direct FP64 evaluation of ||B - A X_30||
+    b = _bp(hip, g, proj, vol).double()
+    r = b - _apply_a(hip, g, proj, x).double()
+    direct = float(torch.linalg.norm(r) / torch.linalg.norm(b))
+    print(f"C3 FP32: ||B - A X_30|| / ||B|| direct {direct:.6g}, recursion {res[-1]:.6g}")
+    assert abs(direct / res[-1] - 1.) <= 1e-2
+    assert abs(float(torch.linalg.norm(b)) / float(fx["b_norm"]) - 1.) <= 1e-5
+
+
+def test_c3_first_step_is_alpha0_times_b(hip, c3):
+    g, proj, vol, _ = c3
+    x1, _ = _cg(hip, c3, 1)
+    b = _bp(hip, g, proj, vol)
+    ab = _apply_a(hip, g, proj, vol.like(b))
+    bd = b.double()
+    alpha0 = float(torch.dot(bd.ravel(), bd.ravel()) / torch.dot(bd.ravel(), ab.double().ravel()))
+    e = float(torch.linalg.norm(x1.tensor.double() - alpha0 * bd) / torch.linalg.norm(alpha0 * bd))
+    print(f"C3 first step: alpha0 {alpha0:.6g}, ||X_1 - alpha0 B|| / ||alpha0 B|| {e:.3g}")
+    assert e <= 1e-5
+
+
+@pytest.mark.parametrize("dtype,bar", [("f16", 2e-3), ("bf16", 5e-2)])
+def test_c3_16bit_stacks_vs_fp64_oracle(hip, c3, dtype, bar):
+    """The stacks the backprojector reads (the weighted data once, W FP(P) every
+    iteration) are rounded to FP16 (11 significant bits, relative rounding 2^-12) or
+    BF16 (8 bits, 2^-9).  CG from zero after 30 iterations amplifies a perturbation of
+    the data by the conditioning of the high-frequency components it has started to
+    fit, measured at about 5x to 10x the per-element rounding, so the bars are 2e-3
+    (FP16) and 5e-2 (BF16) relative L2 against the oracle."""
+    _, _, _, fx = c3
+    x, res = _cg(hip, c3, 30, dtype)
+    err = _vs_oracle(x, fx)
+    print(f"C3 {dtype} stacks: X_30 vs oracle {err}, last residual {res[-1]:.6g} (oracle {fx['residual'][-1]:.6g})")
+    assert max(err.values()) <= bar

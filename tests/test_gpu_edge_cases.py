@@ -131,3 +131,68 @@ def test_zero_views_at_kernel_level(hip):
     _lib.check(L.rtkhip_displaced_detector_weight(pd, 16, 8, g4p, 5., 1, -10., 1., pv, vv, None))
     torch.cuda.synchronize()
     assert torch.all(vol == 7.) and torch.all(proj == 3.)
+
+
+@pytest.mark.parametrize("dtype", [torch.float16, torch.bfloat16])
+def test_filters_reject_16bit_images_where_the_library_takes_fp32(hip, dtype):
+    """Volumes, and the stacks of the FDK, the forward projector, CG and the weighting
+    filters, are FP32 buffers in the C ABI; only the backprojectors read 16-bit stacks.
+    A 16-bit image elsewhere raises ValueError before any library call."""
+    rows = ROWS[:4]
+    g = hip_geometry(hip, rows)
+    size, org, sp = _stack_img(24, 20, 4)
+    p32 = hip.ConstantImageSource(size, sp, org, constant=1.)
+    v32 = hip.ConstantImageSource(VSIZE, VSP, VORG)
+    p16, v16 = p32.like(p32.tensor.to(dtype)), v32.like(v32.tensor.to(dtype))
+    cases = [(hip.HipFDKConeBeamReconstructionFilter, (v16, p32)), (hip.HipFDKConeBeamReconstructionFilter, (v32, p16)),
+             (hip.HipForwardProjectionImageFilter, (p16, v32)), (hip.HipForwardProjectionImageFilter, (p32, v16)),
+             (hip.HipBackProjectionImageFilter, (v16, p32)), (hip.HipJosephBackProjectionImageFilter, (v16, p32)),
+             (hip.HipConjugateGradientConeBeamReconstructionFilter, (v16, p32)),
+             (hip.HipConjugateGradientConeBeamReconstructionFilter, (v32, p16))]
+    for cls, (a, b) in cases:
+        f = cls()
+        f.SetInput(0, a)
+        f.SetInput(1, b)
+        f.SetGeometry(g)
+        with pytest.raises(ValueError):
+            f.Update()
+    for cls in (hip.HipParkerShortScanImageFilter, hip.HipDisplacedDetectorImageFilter):
+        f = cls()
+        f.SetInput(p16)
+        f.SetGeometry(g)
+        with pytest.raises(ValueError):
+            f.Update()
+    with pytest.raises(ValueError):
+        hip.draw_quadrics(v16, hip.phantoms.shepp_logan(20.))
+    with pytest.raises(ValueError):
+        hip.project_quadrics(g, p16, hip.phantoms.shepp_logan(20.))
+    # the backprojectors do read 16-bit stacks
+    for cls in (hip.HipBackProjectionImageFilter, hip.HipJosephBackProjectionImageFilter):
+        f = cls()
+        f.SetInput(0, v32)
+        f.SetInput(1, p16)
+        f.SetGeometry(g)
+        f.Update()
+        assert torch.isfinite(f.GetOutput().tensor).all()
+
+
+def test_cg_weights_must_match_the_projections(hip):
+    """The CG weights input is read as one float per projection pixel: a weights image of
+    another size (e.g. a single view for a four-view stack) or dtype is refused."""
+    g = hip_geometry(hip, ROWS[:4])
+    size, org, sp = _stack_img(24, 20, 4)
+    p = hip.ConstantImageSource(size, sp, org, constant=1.)
+    v = hip.ConstantImageSource(VSIZE, VSP, VORG)
+    f = hip.HipConjugateGradientConeBeamReconstructionFilter()
+    f.SetInput(0, v)
+    f.SetInput(1, p)
+    f.SetGeometry(g)
+    f.SetNumberOfIterations(1)
+    with pytest.raises(ValueError):
+        f.SetInputWeights(p.like(p.tensor.half()))
+    f.SetInputWeights(hip.ConstantImageSource((24, 20, 1), sp, org, constant=1.))
+    with pytest.raises(ValueError):
+        f.Update()
+    f.SetInputWeights(hip.ConstantImageSource(size, sp, org, constant=1.))
+    f.Update()
+    assert torch.isfinite(f.GetOutput().tensor).all()
