@@ -163,8 +163,10 @@ int rtkhip_displaced_detector_weight(const int in_dim[3], int out_nu, int in_off
  * (centred, optional Ohnesorge truncation mirror), R2C FFT, multiplication by
  * the kernel spectrum, C2R with 1/xpad, crop.  Replaces CUDA_padding +
  * CUDA_fft_convolution + CudaCropImageFilter
- * (src/rtkCudaFFTProjectionsConvolutionImageFilter.cu:70-181) for 1-D kernels.
- *   xpad       power of two, 2*Nu <= xpad <= 8192
+ * (src/rtkCudaFFTProjectionsConvolutionImageFilter.cu:70-181) for 1-D kernels, and
+ * their row part for the 2-D kernel (then rtkhip_column_filter).
+ *   xpad       Nu <= xpad <= 8192, no prime factor above 13 (RTK pads to 2 Nu made
+ *              2- or 13-smooth)
  *   kernel     (xpad/2+1) complex values as interleaved doubles (re, im)
  *   trunc_next number of mirrored pixels per side (0 = none); mirror_weights has
  *              trunc_next+1 doubles (weights[b] for border distance b)
@@ -174,6 +176,20 @@ int rtkhip_ramp_filter(const int proj_dim[3], int xpad, const double * kernel, i
                        const double * mirror_weights, const double * weight_geom7, const double proj_origin[2],
                        const double proj_spacing[2], const float * dev_in, void * dev_out, int out_dtype,
                        void * stream);
+
+/* Filtering of every detector column (along v) of a projection slab: each column is
+ * zero padded at its end to `ypad` samples, transformed, multiplied by the spectrum,
+ * inverse transformed with 1/ypad and cropped to Nv.  RTK's 2-D ramp kernel
+ * (HannCutFrequencyY, include/rtkFFTRampImageFilter.hxx:143-176; on CUDA
+ * multiply_kernel2D, src/rtkCudaFFTProjectionsConvolutionImageFilter.cu:49-68) is the
+ * outer product of the 1-D ramp row and a window along v, so the 2-D convolution is
+ * rtkhip_ramp_filter (row kernel) followed by this filter (the window).
+ *   ypad       Nv <= ypad <= 8192, no prime factor above 13
+ *   kernel     ypad complex values (re, im interleaved doubles): the full column
+ *              spectrum, Hermitian (K[ypad - k] = conj K[k]) for a real kernel
+ *   dev_in     FP32; dev_out of enum rtkhip_dtype out_dtype; in == out allowed for FP32 */
+int rtkhip_column_filter(const int proj_dim[3], int ypad, const double * kernel, const float * dev_in,
+                         void * dev_out, int out_dtype, void * stream);
 
 /* One CG step of include/rtkConjugateGradientImageFilter.hxx:166-250 on device
  * vectors of n floats, given AP = A*P (replaces CUDA_conjugate_gradient,
@@ -193,6 +209,36 @@ int rtkhip_cg_init(size_t n, const float * dev_b, const float * dev_ax, float * 
  * to *dev_result (device double). */
 int rtkhip_dot(size_t n, const float * dev_a, const float * dev_b, double * dev_result, void * stream);
 
+/* dev_out = dev_a * dev_b element-wise over n floats (aliasing allowed): the projection
+ * weighting step of the CG operator, done by an itk::MultiplyImageFilter in RTK
+ * (include/rtkReconstructionConjugateGradientOperator.hxx:216-218). */
+int rtkhip_multiply(size_t n, const float * dev_a, const float * dev_b, float * dev_out, void * stream);
+
+/* ------------------------------------------------------------ image buffers */
+/* Device-resident image buffer with an optional host mirror: the image data manager
+ * behind RTK's CUDA filters (itk::CudaImage / CudaDataManager of ITKCudaCommon, whose
+ * device pointers the filters take with GetCudaDataManager()->GetGPUBufferPointer(),
+ * src/rtkCudaFDKBackProjectionImageFilter.cxx:71-73).  The buffer owns one device
+ * allocation of `bytes`; the caller may attach host memory of the same size (a CPU
+ * image's pixels, still owned by the caller).  Each access makes the accessed side
+ * current first (one copy if the other side was written since) and a write access
+ * marks the other side stale, so images passed between filters stay in HBM.  Calls on
+ * one buffer are serialised; transfers are ordered on `stream`. */
+typedef struct rtkhip_buffer rtkhip_buffer;
+rtkhip_buffer * rtkhip_buffer_create(size_t bytes); /* NULL on failure (rtkhip_last_error) */
+void            rtkhip_buffer_destroy(rtkhip_buffer * b);
+/* Attaches host memory; its contents become the image (the device copy is stale).
+ * NULL detaches (the device copy stays the image). */
+int rtkhip_buffer_set_host(rtkhip_buffer * b, void * host);
+/* Device address of the image, uploaded first if the host copy is newer; write != 0
+ * marks the host copy stale. */
+int rtkhip_buffer_device(rtkhip_buffer * b, int write, void ** dev, void * stream);
+/* The attached host memory, downloaded first (and `stream` synchronised) if the device
+ * copy is newer; write != 0 marks the device copy stale. */
+int rtkhip_buffer_host(rtkhip_buffer * b, int write, void ** host, void * stream);
+/* Host-to-device and device-to-host copies made so far. */
+int rtkhip_buffer_transfers(const rtkhip_buffer * b, long long * uploads, long long * downloads);
+
 /* ================================================================ filter level */
 
 typedef struct rtkhip_image
@@ -203,18 +249,24 @@ typedef struct rtkhip_image
   double spacing[3];
 } rtkhip_image;
 
-/* Ramp options of rtk::FFTRampImageFilter (include/rtkFFTRampImageFilter.h).
+/* Ramp options of rtk::FFTRampImageFilter (include/rtkFFTRampImageFilter.h) and its
+ * base rtk::FFTProjectionsConvolutionImageFilter (include/rtkFFTProjectionsConvolutionImageFilter.h).
  * Only one window applies, in the order hann, cosine, hamming, ramlak, shepp_logan. */
 typedef struct rtkhip_ramp_options
 {
   double hann_cut;
-  double hann_cut_y; /* must be 0 (2-D kernels are not supported yet) */
+  double hann_cut_y; /* > 0: the 2-D kernel (ramp row x Hann window along v) */
   double cosine_cut;
   double hamming;
   double ramlak_cut;
   double shepp_logan_cut;
   double truncation_correction;
-  int    zero_pad_x; /* 1 or 2 (default 2) */
+  int    zero_pad_x; /* ZeroPadFactors[0]; <= 0 means RTK's default 2 */
+  int    zero_pad_y; /* ZeroPadFactors[1] (2-D kernel only); <= 0 means 2 */
+  /* GreatestPrimeFactor of the padded sizes, 2..13; 0 means 13, the value of RTK's CUDA
+   * convolution filter (include/rtkCudaFFTProjectionsConvolutionImageFilter.hxx:42).
+   * RTK's CPU filter uses 2 unless ITK's FFT is FFTW. */
+  int greatest_prime_factor;
 } rtkhip_ramp_options;
 
 typedef struct rtkhip_geometry rtkhip_geometry;
@@ -246,6 +298,12 @@ int rtkhip_fdk_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const 
                            const float * dev_vol_in, float * dev_vol_out, const rtkhip_image * stack,
                            const float * dev_proj, const rtkhip_ramp_options * ramp, int subset_size,
                            int stack_dtype, double * bp_ms, void * stream);
+
+/* rtk::HipFFTRampImageFilter (rtk::FFTRampImageFilter, include/rtkFFTRampImageFilter.hxx:47-178,
+ * on the padding of include/rtkFFTProjectionsConvolutionImageFilter.hxx:203-304):
+ * dev_out = ramp-filtered dev_in (FP32 both; in == out allowed).  `ramp` NULL = defaults. */
+int rtkhip_fft_ramp_filter(rtkhip_context * c, const rtkhip_image * stack, const float * dev_in, float * dev_out,
+                           const rtkhip_ramp_options * ramp, void * stream);
 
 /* rtk::HipForwardProjectionImageFilter (Joseph).  proj_out = proj_in + FP(vol). */
 int rtkhip_forward_project(rtkhip_context * c, const rtkhip_geometry * g, const rtkhip_image * stack,

@@ -14,6 +14,7 @@ from .filters import (  # noqa: F401
     HipDisplacedDetectorImageFilter,
     HipFDKBackProjectionImageFilter,
     HipFDKConeBeamReconstructionFilter,
+    HipFFTRampImageFilter,
     HipForwardProjectionImageFilter,
     HipJosephBackProjectionImageFilter,
     HipParkerShortScanImageFilter,

@@ -39,7 +39,8 @@ class RampOptions(ctypes.Structure):
     _fields_ = [("hann_cut", ctypes.c_double), ("hann_cut_y", ctypes.c_double),
                 ("cosine_cut", ctypes.c_double), ("hamming", ctypes.c_double),
                 ("ramlak_cut", ctypes.c_double), ("shepp_logan_cut", ctypes.c_double),
-                ("truncation_correction", ctypes.c_double), ("zero_pad_x", ctypes.c_int)]
+                ("truncation_correction", ctypes.c_double), ("zero_pad_x", ctypes.c_int),
+                ("zero_pad_y", ctypes.c_int), ("greatest_prime_factor", ctypes.c_int)]
 
 
 P = ctypes.c_void_p
@@ -60,9 +61,17 @@ SIGNATURES = [
     ("rtkhip_joseph_backproject", I, [PI3, PI3, PD, PD, PD, D, D, P, P, P, I, P]),
     ("rtkhip_fdk_weight", I, [PI3, PD, PD, PD, P, P, P]),
     ("rtkhip_ramp_filter", I, [PI3, I, PD, I, PD, PD, PD, PD, P, P, I, P]),
+    ("rtkhip_column_filter", I, [PI3, I, PD, P, P, I, P]),
     ("rtkhip_cg_step", I, [ctypes.c_size_t, P, P, P, P, P, P]),
     ("rtkhip_cg_init", I, [ctypes.c_size_t, P, P, P, P, P, P]),
     ("rtkhip_dot", I, [ctypes.c_size_t, P, P, P, P]),
+    ("rtkhip_multiply", I, [ctypes.c_size_t, P, P, P, P]),
+    ("rtkhip_buffer_create", P, [ctypes.c_size_t]),
+    ("rtkhip_buffer_destroy", None, [P]),
+    ("rtkhip_buffer_set_host", I, [P, P]),
+    ("rtkhip_buffer_device", I, [P, I, ctypes.POINTER(P), P]),
+    ("rtkhip_buffer_host", I, [P, I, ctypes.POINTER(P), P]),
+    ("rtkhip_buffer_transfers", I, [P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
     ("rtkhip_geometry_create", P, []),
     ("rtkhip_geometry_destroy", None, [P]),
     ("rtkhip_geometry_add_projection", I, [P, D, D, D, D, D, D, D, D, D]),
@@ -73,6 +82,7 @@ SIGNATURES = [
     ("rtkhip_context_destroy", None, [P]),
     ("rtkhip_fdk_reconstruct", I, [P, P, ctypes.POINTER(Image), P, P, ctypes.POINTER(Image), P,
                                    ctypes.POINTER(RampOptions), I, I, PD, P]),
+    ("rtkhip_fft_ramp_filter", I, [P, ctypes.POINTER(Image), P, P, ctypes.POINTER(RampOptions), P]),
     ("rtkhip_forward_project", I, [P, P, ctypes.POINTER(Image), P, P, ctypes.POINTER(Image), P,
                                    ctypes.POINTER(ctypes.c_longlong), P]),
     ("rtkhip_back_project", I, [P, P, ctypes.POINTER(Image), P, P, ctypes.POINTER(Image), P, I, I, P]),

@@ -310,6 +310,7 @@ struct rtkhip_geometry
 struct rtkhip_context
 {
   rtkhip::DeviceBuffer<unsigned char> filtered;
+  rtkhip::DeviceBuffer<float>         rowFiltered; // FP32 row-filtered slab before the 2-D kernel's column pass
   rtkhip::DeviceBuffer<float>         vB, vR, vP, vAP, pT;
   rtkhip::DeviceBuffer<unsigned short> pT16;   // 16-bit stack read by the CG backprojector
   rtkhip::DeviceBuffer<unsigned>       maxBits; // max |stack| for the FP16 scaling
@@ -477,29 +478,54 @@ WeightGeometry(const rtkhip_geometry & g, const rtkhip_image & stack)
   return out;
 }
 
-// In-place complex FFT (radix-2, power-of-two n), forward sign -1.
+// Forward DFT of any length in FP64, for the ramp kernel's spectrum (once per padded
+// size).  Mixed-radix decimation in time: the smallest prime factor p of n splits the
+// sequence into p interleaved subsequences whose transforms are combined with the
+// twiddles exp(-2 pi i r k / n); a prime length is summed directly.
 void
 HostFFT(std::vector<std::complex<double>> & a)
 {
   const int n = int(a.size());
-  for (int i = 1, j = 0; i < n; i++)
+  if (n <= 1)
+    return;
+  int p = 2;
+  while (p * p <= n && n % p)
+    p++;
+  if (n % p)
+    p = n;
+  auto tw = [n](long e) {
+    const double ang = -2 * kPi * double(e % n) / n;
+    return std::complex<double>(std::cos(ang), std::sin(ang));
+  };
+  std::vector<std::complex<double>> out(n);
+  if (p == n)
   {
-    int bit = n >> 1;
-    for (; j & bit; bit >>= 1)
-      j ^= bit;
-    j ^= bit;
-    if (i < j)
-      std::swap(a[i], a[j]);
+    for (int k = 0; k < n; k++)
+    {
+      std::complex<double> acc = 0.;
+      for (int r = 0; r < n; r++)
+        acc += a[r] * tw(long(r) * k);
+      out[k] = acc;
+    }
+    a.swap(out);
+    return;
   }
-  for (int len = 2; len <= n; len <<= 1)
-    for (int i = 0; i < n; i += len)
-      for (int k = 0; k < len / 2; k++)
-      {
-        const double               ang = -2 * kPi * k / len;
-        const std::complex<double> w(std::cos(ang), std::sin(ang)), x = a[i + k], y = a[i + k + len / 2] * w;
-        a[i + k] = x + y;
-        a[i + k + len / 2] = x - y;
-      }
+  const int                                      m = n / p;
+  std::vector<std::vector<std::complex<double>>> sub(p, std::vector<std::complex<double>>(m));
+  for (int r = 0; r < p; r++)
+  {
+    for (int q = 0; q < m; q++)
+      sub[r][q] = a[size_t(q) * p + r];
+    HostFFT(sub[r]);
+  }
+  for (int k = 0; k < n; k++)
+  {
+    std::complex<double> acc = 0.;
+    for (int r = 0; r < p; r++)
+      acc += sub[r][k % m] * tw(long(r) * k);
+    out[k] = acc;
+  }
+  a.swap(out);
 }
 
 // Ramp kernel spectrum (xpad/2+1 complex, interleaved re/im).
@@ -561,6 +587,61 @@ RampKernel(rtkhip_context & c, int xpad, double spacing, const rtkhip_ramp_optio
     out[2 * q + 1] = k[q].imag();
   }
   return c.rampKernels.emplace(key, std::move(out)).first->second;
+}
+
+// GreatestPrimeFactor of include/rtkFFTProjectionsConvolutionImageFilter.hxx:329-340
+// (2 for n <= 1).
+int
+GreatestPrimeFactor(int n)
+{
+  int v = 2;
+  while (v <= n)
+  {
+    if (n % v == 0)
+      n /= v;
+    else
+      v++;
+  }
+  return v;
+}
+
+// Padded sizes of include/rtkFFTProjectionsConvolutionImageFilter.hxx:280-304: the
+// smallest size >= ZeroPadFactor * N whose greatest prime factor is at most the
+// configured one, along u; along v the same with the y factor only for a 2-D kernel
+// (HannCutFrequencyY), else Nv made smooth.  Unset (zero) options take RTK's defaults:
+// zero-pad factors 2, and greatest prime factor 13, the value of RTK's CUDA convolution
+// filter (include/rtkCudaFFTProjectionsConvolutionImageFilter.hxx:42) that these
+// filters replace (RTK's CPU filter uses 2 unless its ITK FFT is FFTW).
+void
+PaddedSizes(int nu, int nv, const rtkhip_ramp_options & o, int & xpad, int & ypad)
+{
+  const int gpf = o.greatest_prime_factor == 0 ? 13 : o.greatest_prime_factor;
+  RTKHIP_REQUIRE(gpf >= 2 && gpf <= 13, RTKHIP_ERR_UNSUPPORTED,
+                 "ramp: greatest prime factor must lie in [2, 13] (the FFT radices available)");
+  const int zpx = o.zero_pad_x <= 0 ? 2 : o.zero_pad_x, zpy = o.zero_pad_y <= 0 ? 2 : o.zero_pad_y;
+  RTKHIP_REQUIRE(zpx <= 8 && zpy <= 8, RTKHIP_ERR_INVALID_ARGUMENT, "ramp: zero-pad factors above 8");
+  xpad = zpx * nu;
+  while (GreatestPrimeFactor(xpad) > gpf)
+    xpad++;
+  ypad = nv * (o.hann_cut_y > 0. ? zpy : 1);
+  while (GreatestPrimeFactor(ypad) > gpf)
+    ypad++;
+}
+
+// Column spectrum of the 2-D kernel of include/rtkFFTRampImageFilter.hxx:143-176: the
+// reference replicates the ramp row over the padded height with a Hann window on the
+// first ncut rows and their mirror images, so the 2-D kernel is the outer product of the
+// row kernel and this window (ypad complex values, interleaved; imaginary parts 0).
+std::vector<double>
+HannYSpectrum(int ypad, double cut)
+{
+  const int           ncut = int(std::floor((ypad / 2 + 1) * std::min(1.0, cut) + 0.5));
+  std::vector<double> w(2 * size_t(ypad), 0.);
+  for (int j = 0; j < ncut; j++)
+    w[2 * j] = 0.5 * (1 + std::cos(kPi * j / ncut));
+  for (int j = 1; j < ncut; j++)
+    w[2 * (ypad - j)] = 0.5 * (1 + std::cos(kPi * j / ncut));
+  return w;
 }
 
 __global__ void
@@ -847,6 +928,41 @@ rtkhip_forward_project(rtkhip_context * c, const rtkhip_geometry * g, const rtkh
 }
 
 extern "C" int
+rtkhip_fft_ramp_filter(rtkhip_context * c, const rtkhip_image * stack, const float * dev_in, float * dev_out,
+                       const rtkhip_ramp_options * ramp, void * stream)
+{
+  return Guard([&] {
+    RTKHIP_REQUIRE(c && dev_in && dev_out, RTKHIP_ERR_INVALID_ARGUMENT, "fft_ramp_filter: null argument");
+    CheckImage(stack, "fft_ramp_filter projections");
+    rtkhip_ramp_options o{};
+    if (ramp)
+      o = *ramp;
+    RequireDevice();
+    const int nu = stack->size[0], nv = stack->size[1], np = stack->size[2];
+    int       xpad = 0, ypad = 0;
+    PaddedSizes(nu, nv, o, xpad, ypad);
+    const long zeroext = (long(xpad) - nu) / 2;
+    // truncation extent floor(TruncationCorrection * Nu), clipped by the zero padding
+    // (include/rtkFFTProjectionsConvolutionImageFilter.hxx:210-211,345-355)
+    const int           nextFull = int(std::floor(o.truncation_correction * nu));
+    const int           next = int(std::min<long>(zeroext, nextFull));
+    std::vector<double> mirror(nextFull + 1);
+    for (int i = 0; i <= nextFull; i++)
+      mirror[i] = std::pow(std::sin((nextFull - i) * kPi / (2 * nextFull - 2)), 0.75);
+    const auto & kernel = RampKernel(*c, xpad, stack->spacing[0], o);
+    const int    pd[3] = { nu, nv, np };
+    Throw(rtkhip_ramp_filter(pd, xpad, kernel.data(), next, mirror.data(), nullptr, nullptr, nullptr, dev_in, dev_out,
+                             RTKHIP_F32, stream));
+    if (o.hann_cut_y > 0.)
+    {
+      const std::vector<double> wy = HannYSpectrum(ypad, o.hann_cut_y);
+      Throw(rtkhip_column_filter(pd, ypad, wy.data(), dev_out, dev_out, RTKHIP_F32, stream));
+    }
+    RTKHIP_CHECK(hipStreamSynchronize(Stream(stream)));
+  });
+}
+
+extern "C" int
 rtkhip_fdk_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const rtkhip_image * vol,
                        const float * dev_vol_in, float * dev_vol_out, const rtkhip_image * stack, const float * dev_proj,
                        const rtkhip_ramp_options * ramp, int subset_size, int stack_dtype, double * bp_ms,
@@ -860,19 +976,15 @@ rtkhip_fdk_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const rtkh
     RTKHIP_REQUIRE(stack_dtype == RTKHIP_F32 || stack_dtype == RTKHIP_F16 || stack_dtype == RTKHIP_BF16,
                    RTKHIP_ERR_INVALID_ARGUMENT, "fdk_reconstruct: unknown stack dtype");
     rtkhip_ramp_options o{};
-    o.zero_pad_x = 2;
     if (ramp)
       o = *ramp;
-    RTKHIP_REQUIRE(o.hann_cut_y == 0., RTKHIP_ERR_UNSUPPORTED, "HannCutFrequencyY (2-D ramp kernel) is not supported");
-    const int zp = std::min(std::max(o.zero_pad_x, 1), 2);
     RequireDevice();
     hipStream_t s = Stream(stream);
     const int   nu = stack->size[0], nv = stack->size[1], np = stack->size[2];
-    // Padded size: smallest power of two >= zp*Nu (greatest prime factor 2,
-    // include/rtkFFTProjectionsConvolutionImageFilter.hxx:285-288).
-    int xpad = 1;
-    while (xpad < zp * nu)
-      xpad <<= 1;
+    int         xpad = 0, ypad = 0;
+    PaddedSizes(nu, nv, o, xpad, ypad);
+    const bool                twoD = o.hann_cut_y > 0.;
+    const std::vector<double> wy = twoD ? HannYSpectrum(ypad, o.hann_cut_y) : std::vector<double>();
     const long zeroext = (long(xpad) - nu) / 2;
     const int  nextFull = int(std::floor(o.truncation_correction * nu));
     const int  next = int(std::min<long>(zeroext, nextFull));
@@ -900,6 +1012,8 @@ rtkhip_fdk_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const rtkh
       return;
     }
     c->filtered.Resize(ppix * subset * DtypeSize(stack_dtype));
+    if (twoD && stack_dtype != RTKHIP_F32)
+      c->rowFiltered.Resize(ppix * subset);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (bp_ms)
     {
@@ -913,8 +1027,18 @@ rtkhip_fdk_reconstruct(rtkhip_context * c, const rtkhip_geometry * g, const rtkh
       const int cnt = std::min(subset, np - first);
       const int pd[3] = { nu, nv, cnt };
       const int vd[3] = { vol->size[0], vol->size[1], vol->size[2] };
-      Throw(rtkhip_ramp_filter(pd, xpad, kernel.data(), next, mirror.data(), w7all.data() + 7 * size_t(first), origin,
-                               stack->spacing, dev_proj + ppix * first, c->filtered.get(), stack_dtype, stream));
+      if (twoD)
+      {
+        // rows into FP32, then the column window into the stack type
+        float * rows = stack_dtype == RTKHIP_F32 ? reinterpret_cast<float *>(c->filtered.get()) : c->rowFiltered.get();
+        Throw(rtkhip_ramp_filter(pd, xpad, kernel.data(), next, mirror.data(), w7all.data() + 7 * size_t(first),
+                                 origin, stack->spacing, dev_proj + ppix * first, rows, RTKHIP_F32, stream));
+        Throw(rtkhip_column_filter(pd, ypad, wy.data(), rows, c->filtered.get(), stack_dtype, stream));
+      }
+      else
+        Throw(rtkhip_ramp_filter(pd, xpad, kernel.data(), next, mirror.data(), w7all.data() + 7 * size_t(first),
+                                 origin, stack->spacing, dev_proj + ppix * first, c->filtered.get(), stack_dtype,
+                                 stream));
       if (bp_ms)
         RTKHIP_CHECK(hipEventRecord(e0, s));
       Throw(rtkhip_fdk_backproject(pd, vd, mats.data() + 12 * size_t(first), border.data() + first, vin, dev_vol_out,

@@ -19,6 +19,7 @@
 #include <hip/hip_fp16.h>
 
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -513,6 +514,273 @@ __global__ __launch_bounds__(64) void ramp_wave_kernel(const float * __restrict_
   StockhamFFT<M, true>(lload, gstore, lload, lstore, tw, lane);
 }
 
+// ------------------------------------------------------- mixed-radix FFTs --
+// Padded lengths whose prime factors are all at most 13: RTK pads to such lengths
+// when its FFT is FFTW, and its CUDA convolution filter always does (greatest prime
+// factor 13, include/rtkCudaFFTProjectionsConvolutionImageFilter.hxx:42).  These run
+// as complex Stockham autosort FFTs in LDS (the same pass as StockhamPass, with the
+// butterfly's radix chosen at run time from 2, 3, 4, 5, 7, 11, 13 and the R-point DFT
+// done directly from the R roots of unity held in registers), ping-ponging between two
+// LDS buffers.  A filtered row or column is real, so two of them travel as the real and
+// imaginary parts of one complex sequence: for a real kernel h the spectrum K is
+// Hermitian, and IFFT(K FFT(a + i b)) = (h * a) + i (h * b) exactly.
+constexpr int kMaxRadices = 24;
+
+// One pass of radix R over `batch` transforms of N points stored back to back:
+// butterfly j reads src[j + r N/R], multiplies by W_{NS R}^{r (j mod NS)} (tw[t] =
+// exp(-2 pi i t / N)), applies the R-point DFT and writes dst[(j - j mod NS) R + j mod NS
+// + r NS].
+template <int R, bool INV>
+__device__ __forceinline__ void
+MixedPass(const float2 * __restrict__ src, float2 * __restrict__ dst, const float2 * __restrict__ tw, int N, int NS,
+          int batch)
+{
+  const int NR = N / R, step = N / (NS * R);
+  float2    root[R];
+#pragma unroll
+  for (int m = 0; m < R; m++)
+    root[m] = INV ? conj2(tw[m * NR]) : tw[m * NR];
+  for (int jj = threadIdx.x; jj < batch * NR; jj += blockDim.x)
+  {
+    const int      f = jj / NR, j = jj - f * NR, k = j % NS;
+    const float2 * sp = src + f * N;
+    float2 *       dp = dst + f * N;
+    float2         v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      v[r] = sp[j + r * NR];
+#pragma unroll
+    for (int r = 1; r < R; r++)
+      v[r] = cmul(v[r], INV ? conj2(tw[r * k * step]) : tw[r * k * step]);
+    const int o = (j - k) * R + k;
+#pragma unroll
+    for (int q = 0; q < R; q++)
+    {
+      float2 acc = v[0];
+#pragma unroll
+      for (int r = 1; r < R; r++)
+      {
+        const float2 t = cmul(v[r], root[(r * q) % R]);
+        acc.x += t.x;
+        acc.y += t.y;
+      }
+      dp[o + q * NS] = acc;
+    }
+  }
+}
+
+// `batch` N-point FFTs of a (b is the other buffer); returns the buffer holding the
+// result.  The caller synchronises before (input written) and after (result read).
+template <bool INV>
+__device__ float2 *
+MixedFFT(float2 * a, float2 * b, const float2 * __restrict__ tw, int N, const int * __restrict__ rad, int nrad,
+         int batch)
+{
+  int NS = 1;
+  for (int s = 0; s < nrad; s++)
+  {
+    const int R = rad[s];
+    switch (R)
+    {
+      case 2:
+        MixedPass<2, INV>(a, b, tw, N, NS, batch);
+        break;
+      case 3:
+        MixedPass<3, INV>(a, b, tw, N, NS, batch);
+        break;
+      case 4:
+        MixedPass<4, INV>(a, b, tw, N, NS, batch);
+        break;
+      case 5:
+        MixedPass<5, INV>(a, b, tw, N, NS, batch);
+        break;
+      case 7:
+        MixedPass<7, INV>(a, b, tw, N, NS, batch);
+        break;
+      case 11:
+        MixedPass<11, INV>(a, b, tw, N, NS, batch);
+        break;
+      default:
+        MixedPass<13, INV>(a, b, tw, N, NS, batch);
+        break;
+    }
+    __syncthreads();
+    float2 * t = a;
+    a = b;
+    b = t;
+    NS *= R;
+  }
+  return a;
+}
+
+// Sample n of a weighted, padded row (zero padding, optional Ohnesorge mirror), as the
+// LDS and wavefront kernels build it.
+__device__ __forceinline__ float
+PaddedSample(const float * row, const WeightView * wv, float dx, float dy, int j, int nu, int zeroext, int next,
+             const float * mirror, int n)
+{
+  auto      W = [&](int i) -> float { return wv ? row[i] * CosGammaWeight(*wv, dx, dy, i, j) : row[i]; };
+  const int i = n - zeroext;
+  if (i >= 0 && i < nu)
+    return W(i);
+  if (next > 0)
+  {
+    if (i < 0 && -i <= next)
+      return mirror[-i] * (2.f * W(1) - W(-i));
+    if (i >= nu && i - (nu - 1) <= next)
+    {
+      const int b = i - (nu - 1);
+      return mirror[b] * (2.f * W(nu - 1) - W(nu - 1 - b));
+    }
+  }
+  return 0.f;
+}
+
+// Ramp filter of two detector rows per workgroup (rows 2 blockIdx.x and 2 blockIdx.x + 1
+// of the slab, in (view, row) order) for any padded length N with prime factors <= 13.
+// K: the full N-point kernel spectrum.  LDS: 2N complex values.
+template <typename TO>
+__global__ __launch_bounds__(256) void ramp_mixed_kernel(const float * __restrict__ in, TO * __restrict__ out, int nu,
+                                                          int nv, long nrows, int N, int zeroext,
+                                                          const float2 * __restrict__ tw,
+                                                          const float2 * __restrict__ K, const int * __restrict__ rad,
+                                                          int nrad, int next, const float * __restrict__ mirror,
+                                                          const WeightView * __restrict__ wviews, float dx, float dy)
+{
+  extern __shared__ float2 lds[];
+  const long r0 = 2 * long(blockIdx.x);
+  const bool two = r0 + 1 < nrows;
+  const int  p0 = int(r0 / nv), j0 = int(r0 % nv);
+  const int  p1 = two ? int((r0 + 1) / nv) : p0, j1 = two ? int((r0 + 1) % nv) : j0;
+  const float * row0 = in + r0 * nu;
+  const float * row1 = in + (two ? r0 + 1 : r0) * nu;
+  const WeightView * wv0 = wviews ? wviews + p0 : nullptr;
+  const WeightView * wv1 = wviews ? wviews + p1 : nullptr;
+  for (int n = threadIdx.x; n < N; n += blockDim.x)
+    lds[n] = make_float2(PaddedSample(row0, wv0, dx, dy, j0, nu, zeroext, next, mirror, n),
+                         two ? PaddedSample(row1, wv1, dx, dy, j1, nu, zeroext, next, mirror, n) : 0.f);
+  __syncthreads();
+  float2 * z = MixedFFT<false>(lds, lds + N, tw, N, rad, nrad, 1);
+  float2 * other = z == lds ? lds + N : lds;
+  for (int k = threadIdx.x; k < N; k += blockDim.x)
+    z[k] = cmul(z[k], K[k]);
+  __syncthreads();
+  z = MixedFFT<true>(z, other, tw, N, rad, nrad, 1);
+  const float scale = 1.f / float(N);
+  TO *        o0 = out + r0 * nu;
+  TO *        o1 = out + (r0 + 1) * nu;
+  for (int i = threadIdx.x; i < nu; i += blockDim.x)
+  {
+    const float2 v = z[zeroext + i];
+    StoreOut(o0, i, v.x * scale);
+    if (two)
+      StoreOut(o1, i, v.y * scale);
+  }
+}
+
+// Filters every detector column (along v) of view blockIdx.y, columns 2F blockIdx.x ..
+// 2F blockIdx.x + 2F - 1: each column is zero padded at its end to N samples, circularly
+// convolved through its N-point spectrum with K (full spectrum) and cropped back to nv.
+// Columns travel in pairs as complex sequences; F pairs per workgroup, 2 F N complex
+// values of LDS.  Every workgroup reads all of its columns before writing any, and
+// workgroups own disjoint columns, so in == out is allowed (FP32).
+template <typename TO>
+__global__ __launch_bounds__(256) void column_filter_kernel(const float * in, TO * out, int nu, int nv, int N, int F,
+                                                             const float2 * __restrict__ tw,
+                                                             const float2 * __restrict__ K,
+                                                             const int * __restrict__ rad, int nrad)
+{
+  extern __shared__ float2 lds[];
+  const int     p = blockIdx.y, u0 = 2 * F * blockIdx.x, C = 2 * F;
+  const float * vin = in + long(p) * nv * nu;
+  TO *          vout = out + long(p) * nv * nu;
+  float *       ldsf = reinterpret_cast<float *>(lds);
+  for (int e = threadIdx.x; e < N * C; e += blockDim.x)
+  {
+    const int v = e / C, c = e - v * C, u = u0 + c;
+    // pair f = c / 2 of sample v: real part for even c, imaginary for odd c
+    ldsf[2 * ((c >> 1) * N + v) + (c & 1)] = (v < nv && u < nu) ? vin[long(v) * nu + u] : 0.f;
+  }
+  __syncthreads();
+  float2 * z = MixedFFT<false>(lds, lds + F * N, tw, N, rad, nrad, F);
+  float2 * other = z == lds ? lds + F * N : lds;
+  for (int e = threadIdx.x; e < F * N; e += blockDim.x)
+    z[e] = cmul(z[e], K[e % N]);
+  __syncthreads();
+  z = MixedFFT<true>(z, other, tw, N, rad, nrad, F);
+  const float   scale = 1.f / float(N);
+  const float * zf = reinterpret_cast<const float *>(z);
+  for (int e = threadIdx.x; e < nv * C; e += blockDim.x)
+  {
+    const int v = e / C, c = e - v * C, u = u0 + c;
+    if (u < nu)
+      StoreOut(vout, long(v) * nu + u, zf[2 * ((c >> 1) * N + v) + (c & 1)] * scale);
+  }
+}
+
+// Radices of an N-point transform (4s first, then 2, 3, 5, 7, 11, 13); empty if N has
+// a prime factor above 13.
+std::vector<int>
+MixedRadices(int N)
+{
+  std::vector<int> r;
+  while (N % 4 == 0)
+  {
+    r.push_back(4);
+    N /= 4;
+  }
+  for (int q : { 2, 3, 5, 7, 11, 13 })
+    while (N % q == 0)
+    {
+      r.push_back(q);
+      N /= q;
+    }
+  if (N != 1 || r.size() > size_t(kMaxRadices))
+    r.clear();
+  return r;
+}
+
+// Device tables of an N-point mixed-radix filter: tw[0, N) = exp(-2 pi i t / N), then the
+// N-point spectrum K, then the radices (as ints, after the float2 values).
+struct MixedTables
+{
+  DeviceBuffer<float2> buf;
+  int                  nrad = 0;
+  const float2 *       tw() const { return buf.get(); }
+  const float2 *       K(int N) const { return buf.get() + N; }
+  const int *          rad(int N) const { return reinterpret_cast<const int *>(buf.get() + 2 * N); }
+};
+
+// Uploads the tables; spectrum(k) gives K[k] for k in [0, N) as doubles (re, im).
+template <class F>
+void
+UploadMixedTables(MixedTables & t, int N, const std::vector<int> & rad, F && spectrum, hipStream_t s)
+{
+  const double        pi = 3.141592653589793238462643383279502884;
+  const size_t        nr2 = (rad.size() + 1) / 2; // radices packed two per float2 slot
+  std::vector<float2> h(2 * size_t(N) + nr2);
+  for (int q = 0; q < N; q++)
+  {
+    h[q] = make_float2(float(std::cos(-2 * pi * q / N)), float(std::sin(-2 * pi * q / N)));
+    double re = 0., im = 0.;
+    spectrum(q, re, im);
+    h[N + q] = make_float2(float(re), float(im));
+  }
+  std::memcpy(reinterpret_cast<int *>(h.data() + 2 * N), rad.data(), rad.size() * sizeof(int));
+  t.buf.Resize(h.size());
+  t.nrad = int(rad.size());
+  RTKHIP_CHECK(hipMemcpyAsync(t.buf.get(), h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice, s));
+}
+
+size_t
+MaxLds()
+{
+  int dev = 0, maxLds = 0;
+  RTKHIP_CHECK(hipGetDevice(&dev));
+  RTKHIP_CHECK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  return size_t(maxLds);
+}
+
 std::vector<WeightView>
 MakeWeightViews(int np, const double * g7, const double origin[2])
 {
@@ -546,6 +814,7 @@ struct RampCache
   std::map<hipStream_t, DeviceBuffer<float2>>     tables;
   std::map<hipStream_t, DeviceBuffer<float2>>     waveTables;
   std::map<hipStream_t, DeviceBuffer<float>>      mirror;
+  std::map<hipStream_t, MixedTables>              mixedRows, mixedColumns;
 };
 thread_local RampCache g_ramp;
 
@@ -590,8 +859,10 @@ rtkhip_ramp_filter(const int proj_dim[3], int xpad, const double * kernel, int t
     const int nu = proj_dim[0], nv = proj_dim[1], np = proj_dim[2];
     RTKHIP_REQUIRE(nu > 1 && nv > 0 && np >= 0 && np <= 65535 && nv <= 2147483647, RTKHIP_ERR_INVALID_ARGUMENT,
                    "ramp_filter: bad sizes");
-    RTKHIP_REQUIRE(xpad >= 4 && xpad <= 8192 && (xpad & (xpad - 1)) == 0 && xpad >= nu, RTKHIP_ERR_UNSUPPORTED,
-                   "ramp_filter: padded size must be a power of two in [max(4, Nu), 8192]");
+    const bool pow2 = xpad >= 4 && (xpad & (xpad - 1)) == 0;
+    const std::vector<int> radices = pow2 ? std::vector<int>() : MixedRadices(xpad);
+    RTKHIP_REQUIRE(xpad >= 2 && xpad <= 8192 && xpad >= nu && (pow2 || !radices.empty()), RTKHIP_ERR_UNSUPPORTED,
+                   "ramp_filter: padded size must lie in [max(2, Nu), 8192] and have no prime factor above 13");
     RTKHIP_REQUIRE(trunc_next >= 0 && (trunc_next == 0 || (mirror_weights && trunc_next < nu - 1)),
                    RTKHIP_ERR_INVALID_ARGUMENT, "ramp_filter: bad truncation arguments");
     RTKHIP_REQUIRE(out_dtype == RTKHIP_F32 || out_dtype == RTKHIP_F16 || out_dtype == RTKHIP_BF16,
@@ -611,17 +882,6 @@ rtkhip_ramp_filter(const int proj_dim[3], int xpad, const double * kernel, int t
     const long zeroext = (long(xpad) - nu) / 2;
     RTKHIP_REQUIRE(zeroext >= trunc_next, RTKHIP_ERR_INVALID_ARGUMENT, "ramp_filter: mirror wider than padding");
     const double pi = 3.141592653589793238462643383279502884;
-    const int           ntw = (3 * M) / 4;
-    std::vector<float2> tab(ntw + (M + 1) + (M + 1));
-    for (int t = 0; t < ntw; t++)
-      tab[t] = make_float2(float(std::cos(-2 * pi * t / M)), float(std::sin(-2 * pi * t / M)));
-    for (int k = 0; k <= M; k++)
-      tab[ntw + k] = make_float2(float(std::cos(-2 * pi * k / xpad)), float(std::sin(-2 * pi * k / xpad)));
-    for (int k = 0; k <= M; k++)
-      tab[ntw + M + 1 + k] = make_float2(float(kernel[2 * k]), float(kernel[2 * k + 1]));
-    auto & dt = g_ramp.tables[s];
-    dt.Resize(tab.size());
-    RTKHIP_CHECK(hipMemcpyAsync(dt.get(), tab.data(), sizeof(float2) * tab.size(), hipMemcpyHostToDevice, s));
     const float * dmirror = nullptr;
     if (trunc_next > 0)
     {
@@ -645,6 +905,49 @@ rtkhip_ramp_filter(const int proj_dim[3], int xpad, const double * kernel, int t
       dx = float(proj_spacing[0]);
       dy = float(proj_spacing[1]);
     }
+    if (!pow2)
+    {
+      // any 13-smooth length: the full spectrum from the half one (K[N - k] = conj K[k])
+      const int N = xpad;
+      auto &    t = g_ramp.mixedRows[s];
+      UploadMixedTables(
+        t, N, radices,
+        [&](int k, double & re, double & im) {
+          const int  h = k <= N / 2 ? k : N - k;
+          const bool c = k > N / 2;
+          re = kernel[2 * h];
+          im = c ? -kernel[2 * h + 1] : kernel[2 * h + 1];
+        },
+        s);
+      const size_t lds = sizeof(float2) * 2 * size_t(N);
+      RTKHIP_REQUIRE(lds <= MaxLds(), RTKHIP_ERR_UNSUPPORTED, "ramp_filter: padded row does not fit in LDS");
+      const long nrows = long(nv) * np;
+      RTKHIP_REQUIRE((nrows + 1) / 2 <= 2147483647L, RTKHIP_ERR_UNSUPPORTED, "ramp_filter: too many rows");
+      dim3 grid(unsigned((nrows + 1) / 2));
+#define RTKHIP_RAMPM(TO)                                                                                             \
+  hipLaunchKernelGGL(ramp_mixed_kernel<TO>, grid, dim3(256), lds, s, dev_in, static_cast<TO *>(dev_out), nu, nv,    \
+                     nrows, N, int(zeroext), t.tw(), t.K(N), t.rad(N), t.nrad, trunc_next, dmirror, dw, dx, dy)
+      if (out_dtype == RTKHIP_F32)
+        RTKHIP_RAMPM(float);
+      else if (out_dtype == RTKHIP_F16)
+        RTKHIP_RAMPM(__half);
+      else
+        RTKHIP_RAMPM(__hip_bfloat16);
+#undef RTKHIP_RAMPM
+      RTKHIP_CHECK_LAUNCH();
+      return;
+    }
+    const int           ntw = (3 * M) / 4;
+    std::vector<float2> tab(ntw + (M + 1) + (M + 1));
+    for (int t = 0; t < ntw; t++)
+      tab[t] = make_float2(float(std::cos(-2 * pi * t / M)), float(std::sin(-2 * pi * t / M)));
+    for (int k = 0; k <= M; k++)
+      tab[ntw + k] = make_float2(float(std::cos(-2 * pi * k / xpad)), float(std::sin(-2 * pi * k / xpad)));
+    for (int k = 0; k <= M; k++)
+      tab[ntw + M + 1 + k] = make_float2(float(kernel[2 * k]), float(kernel[2 * k + 1]));
+    auto & dt = g_ramp.tables[s];
+    dt.Resize(tab.size());
+    RTKHIP_CHECK(hipMemcpyAsync(dt.get(), tab.data(), sizeof(float2) * tab.size(), hipMemcpyHostToDevice, s));
     if (M == 256 || M == 512 || M == 1024 || M == 2048)
     {
       // wavefront-per-row register FFT; its own table layout
@@ -699,10 +1002,8 @@ rtkhip_ramp_filter(const int proj_dim[3], int xpad, const double * kernel, int t
       return;
     }
     const size_t lds = sizeof(float2) * (2 * M + (3 * M) / 4);
-    int          dev = 0, maxLds = 0;
-    RTKHIP_CHECK(hipGetDevice(&dev));
-    RTKHIP_CHECK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    RTKHIP_REQUIRE(lds <= size_t(maxLds), RTKHIP_ERR_UNSUPPORTED,
+    const size_t maxLds = MaxLds();
+    RTKHIP_REQUIRE(lds <= maxLds, RTKHIP_ERR_UNSUPPORTED,
                    "ramp_filter: padded row does not fit in LDS (" + std::to_string(lds) + " > " +
                      std::to_string(maxLds) + " bytes)");
     dim3         grid(nv, np);
@@ -716,6 +1017,56 @@ rtkhip_ramp_filter(const int proj_dim[3], int xpad, const double * kernel, int t
     else
       RTKHIP_RAMP_LAUNCH(__hip_bfloat16);
 #undef RTKHIP_RAMP_LAUNCH
+    RTKHIP_CHECK_LAUNCH();
+  });
+}
+
+extern "C" int
+rtkhip_column_filter(const int proj_dim[3], int ypad, const double * kernel, const float * dev_in, void * dev_out,
+                     int out_dtype, void * stream)
+{
+  using namespace rtkhip;
+  return Guard([&] {
+    RTKHIP_REQUIRE(proj_dim && kernel && dev_in && dev_out, RTKHIP_ERR_INVALID_ARGUMENT,
+                   "column_filter: null argument");
+    const int nu = proj_dim[0], nv = proj_dim[1], np = proj_dim[2];
+    RTKHIP_REQUIRE(nu > 0 && nv > 0 && np >= 0 && np <= 65535, RTKHIP_ERR_INVALID_ARGUMENT,
+                   "column_filter: bad sizes");
+    const std::vector<int> radices = MixedRadices(ypad);
+    RTKHIP_REQUIRE(ypad >= 2 && ypad <= 8192 && ypad >= nv && !radices.empty(), RTKHIP_ERR_UNSUPPORTED,
+                   "column_filter: padded size must lie in [max(2, Nv), 8192] and have no prime factor above 13");
+    RTKHIP_REQUIRE(out_dtype == RTKHIP_F32 || out_dtype == RTKHIP_F16 || out_dtype == RTKHIP_BF16,
+                   RTKHIP_ERR_INVALID_ARGUMENT, "column_filter: unknown output dtype");
+    RTKHIP_REQUIRE((dev_in != dev_out) || out_dtype == RTKHIP_F32, RTKHIP_ERR_INVALID_ARGUMENT,
+                   "column_filter: in-place filtering needs an FP32 output");
+    RequireDevice();
+    if (np == 0)
+      return;
+    hipStream_t s = Stream(stream);
+    const int   N = ypad;
+    auto &      t = g_ramp.mixedColumns[s];
+    UploadMixedTables(
+      t, N, radices,
+      [&](int k, double & re, double & im) {
+        re = kernel[2 * k];
+        im = kernel[2 * k + 1];
+      },
+      s);
+    // pairs of columns per workgroup: 64 KB of LDS or one pair
+    const int    F = std::max(1, std::min(8, 4096 / N));
+    const size_t lds = sizeof(float2) * 2 * size_t(F) * N;
+    RTKHIP_REQUIRE(lds <= MaxLds(), RTKHIP_ERR_UNSUPPORTED, "column_filter: padded column does not fit in LDS");
+    dim3 grid(unsigned((nu + 2 * F - 1) / (2 * F)), unsigned(np));
+#define RTKHIP_COLF(TO)                                                                                              \
+  hipLaunchKernelGGL(column_filter_kernel<TO>, grid, dim3(256), lds, s, dev_in, static_cast<TO *>(dev_out), nu, nv,  \
+                     N, F, t.tw(), t.K(N), t.rad(N), t.nrad)
+    if (out_dtype == RTKHIP_F32)
+      RTKHIP_COLF(float);
+    else if (out_dtype == RTKHIP_F16)
+      RTKHIP_COLF(__half);
+    else
+      RTKHIP_COLF(__hip_bfloat16);
+#undef RTKHIP_COLF
     RTKHIP_CHECK_LAUNCH();
   });
 }

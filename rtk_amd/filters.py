@@ -216,12 +216,17 @@ class _Filter:
 
 
 class RampFilterOptions:
-    """Options of rtk::FFTRampImageFilter (include/rtkFFTRampImageFilter.h)."""
+    """Options of rtk::FFTRampImageFilter (include/rtkFFTRampImageFilter.h) and of its
+    base rtk::FFTProjectionsConvolutionImageFilter (zero-pad factors, greatest prime
+    factor of the padded sizes, truncation correction)."""
 
     def __init__(self):
         self.hann = self.hannY = self.cosine = self.hamming = self.ramlak = self.shepplogan = 0.
         self.truncation = 0.
-        self.zero_pad = 2
+        self.zero_pad = (2, 2)
+        # 13 as RTK's CUDA convolution filter (rtkCudaFFTProjectionsConvolutionImageFilter.hxx:42);
+        # RTK's CPU filter uses 2 unless ITK's FFT is FFTW
+        self.gpf = 13
 
     def SetHannCutFrequency(self, f):
         self.hann = float(f)
@@ -245,14 +250,48 @@ class RampFilterOptions:
         self.truncation = float(f)
 
     def SetZeroPadFactors(self, f):
-        self.zero_pad = int(f[0]) if hasattr(f, "__len__") else int(f)
+        """(x, y) factors, or one value for both (itk::FixedArray<int, 2> in RTK)."""
+        if hasattr(f, "__len__"):
+            self.zero_pad = (int(f[0]), int(f[1]) if len(f) > 1 else int(f[0]))
+        else:
+            self.zero_pad = (int(f), int(f))
+
+    def SetGreatestPrimeFactor(self, n):
+        if not 2 <= int(n) <= 13:
+            raise ValueError("GreatestPrimeFactor must lie in [2, 13]")
+        self.gpf = int(n)
+
+    def GetGreatestPrimeFactor(self):
+        return self.gpf
 
     def struct(self):
         o = _lib.RampOptions()
         o.hann_cut, o.hann_cut_y, o.cosine_cut = self.hann, self.hannY, self.cosine
         o.hamming, o.ramlak_cut, o.shepp_logan_cut = self.hamming, self.ramlak, self.shepplogan
-        o.truncation_correction, o.zero_pad_x = self.truncation, self.zero_pad
+        o.truncation_correction, o.zero_pad_x, o.zero_pad_y = self.truncation, self.zero_pad[0], self.zero_pad[1]
+        o.greatest_prime_factor = self.gpf
         return o
+
+
+class HipFFTRampImageFilter(_Filter, RampFilterOptions):
+    """rtk::FFTRampImageFilter on HIP (rtk::CudaFFTRampImageFilter's counterpart): ramp
+    filtering of every projection row, with the optional 2-D kernel (HannCutFrequencyY)
+    and truncation correction (include/rtkFFTRampImageFilter.hxx:47-178,
+    include/rtkFFTProjectionsConvolutionImageFilter.hxx:121-304).  Input 0 = projections."""
+
+    def __init__(self):
+        _Filter.__init__(self)
+        RampFilterOptions.__init__(self)
+
+    def Update(self):
+        proj = self._inputs[0]
+        _require_cuda(proj, "ramp projections")
+        out = self._out_for(proj)
+        pd = proj.descriptor()
+        ro = self.struct()
+        check(lib().rtkhip_fft_ramp_filter(_context(), ctypes.byref(pd), _ptr(proj.tensor), _ptr(out), ctypes.byref(ro),
+                                           _stream()))
+        self._output = proj.like(out)
 
 
 class HipFDKConeBeamReconstructionFilter(_Filter):

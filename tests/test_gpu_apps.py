@@ -44,7 +44,7 @@ def test_rtkfdk_cli(dataset, oracle, hip):
     f.Update()
     assert np.array_equal(vol, f.GetOutput().numpy())
     ref = oracle.fdk(ROWS, oracle.image((64,) * 3, (-63.,) * 3, (2.,) * 3), np.zeros((64,) * 3),
-                     oracle.image((NU, NU, N), SORIG + (0.,), SSP + (1.,)), proj.astype(np.float64), hann=0.8)
+                     oracle.image((NU, NU, N), SORIG + (0.,), SSP + (1.,)), proj.astype(np.float64), hann=0.8, gpf=13)
     assert rel_l2(vol, ref) <= 1e-4
 
 

@@ -117,8 +117,8 @@ def test_fdk_shepp_logan_parity_and_reference_thresholds(oracle, hip):
     ov = oracle.image((128,) * 3, (-127.,) * 3, (2.,) * 3)
     os_ = oracle.image((128, 128, 180), (-254.,) * 3, (4.,) * 3)
     p = proj.numpy()
-    ref_f = oracle.fdk(rows, ov, np.zeros((128,) * 3, np.float32), os_, p)
-    ref_d = oracle.fdk(rows, ov, np.zeros((128,) * 3), os_, p.astype(np.float64))
+    ref_f = oracle.fdk(rows, ov, np.zeros((128,) * 3, np.float32), os_, p, gpf=13)
+    ref_d = oracle.fdk(rows, ov, np.zeros((128,) * 3), os_, p.astype(np.float64), gpf=13)
     e_f, e_d = rel_l2(rec, ref_f), rel_l2(rec, ref_d)
     assert e_f <= 1e-4, f"rel L2 vs float oracle {e_f:.3g} (vs double {e_d:.3g})"
     assert e_d <= 1e-4
@@ -147,7 +147,7 @@ def test_fdk_subsets_hann_and_half_stacks(oracle, hip):
         out[name] = f.GetOutput().numpy()
     ov = oracle.image((96,) * 3, (-118.75,) * 3, (2.5,) * 3)
     os_ = oracle.image((128, 128, 60), (-254.,) * 3, (4.,) * 3)
-    ref_d = oracle.fdk(rows, ov, np.zeros((96,) * 3), os_, proj.numpy().astype(np.float64), hann=0.8)
+    ref_d = oracle.fdk(rows, ov, np.zeros((96,) * 3), os_, proj.numpy().astype(np.float64), hann=0.8, gpf=13)
     for name in ("all", "s16", "s7"):
         assert rel_l2(out[name], ref_d) <= 1e-4, name
     assert rel_l2(out["s16"], out["all"]) <= 1e-6
@@ -179,7 +179,7 @@ def test_fdk_noise_hann_reference_thresholds(oracle, hip):
     rec = f.GetOutput().numpy()
     ov = oracle.image((128,) * 3, (-127.,) * 3, (2.,) * 3)
     os_ = oracle.image((128, 128, n), (-254.,) * 3, (4.,) * 3)
-    ref = oracle.fdk(rows, ov, np.zeros((128,) * 3, np.float32), os_, noisy, hann=0.8)
+    ref = oracle.fdk(rows, ov, np.zeros((128,) * 3, np.float32), os_, noisy, hann=0.8, gpf=13)
     assert rel_l2(rec, ref) <= 1e-4
     phantom = oracle.draw_quadrics(ov, np.zeros((128,) * 3, np.float32), q)
     err, psnr, ok = check_image_quality(rec, phantom, 0.72, 22.4, 2.)

@@ -22,8 +22,8 @@ def test_fdk_c1_shape_vs_float_and_double_oracle(oracle, hip):
     proj = proj_d.astype(np.float32)
     vorig = (-63.5,) * 3
     ov = oracle.image((128,) * 3, vorig, (1.,) * 3)
-    ref_f = oracle.fdk(rows, ov, np.zeros((128,) * 3, np.float32), os_, proj, hann=0.8)
-    ref_d = oracle.fdk(rows, ov, np.zeros((128,) * 3), os_, proj.astype(np.float64), hann=0.8)
+    ref_f = oracle.fdk(rows, ov, np.zeros((128,) * 3, np.float32), os_, proj, hann=0.8, gpf=13)
+    ref_d = oracle.fdk(rows, ov, np.zeros((128,) * 3), os_, proj.astype(np.float64), hann=0.8, gpf=13)
     f = hip.HipFDKConeBeamReconstructionFilter()
     f.SetInput(0, hip.ConstantImageSource((128, 128, 128), (1., 1., 1.), vorig, 0.))
     f.SetInput(1, to_device(hip, proj, sorig, ssp))

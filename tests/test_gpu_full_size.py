@@ -63,7 +63,7 @@ def test_fdk_full_size_slabs(oracle, hip, full):
     assert bool(torch.isfinite(out).all())
     for k in SLABS:
         st, ov, band, rb = _slab_case(oracle, rows, proj, k)
-        ref = oracle.fdk(rows, ov, np.zeros((NX, 2, NX), np.float32), st, band, threads=16)
+        ref = oracle.fdk(rows, ov, np.zeros((NX, 2, NX), np.float32), st, band, threads=16, gpf=13)
         got = out[:, k:k + 2, :].cpu().numpy()
         e = rel_l2(got, ref)
         print(f"FDK slab y={k}: rows {rb}, rel L2 {e:.3g}", flush=True)
