@@ -17,10 +17,12 @@
 //   ramp            known answers test/rtkrampfiltertest2.cxx:54-83 (1e-6)
 //   fdk weight      closed form cos(gamma) = SID-ray geometry, brute force
 //   voxel/FDK BP    brute-force counts (constant-1 projections), sum of w^2
-//   Joseph FP       constant-1 volume -> analytic voxel-centre box chord
+//   Joseph FP       constant-1 volume -> analytic voxel-centre box chord; an
+//                   independent numpy brute force on tiny random volumes (all three
+//                   dominant axes, partial end steps, clamped border taps, 1e-12)
 //   CG              SPD invariants, monotone residual, double vs float
 //   phantoms        Shepp-Logan table src/rtkSheppLoganPhantom.cxx:23-35,
-//                   chord length of a sphere in closed form
+//                   chord length of a sphere in closed form, SetEllipsoid's y rotation
 //   truncation      padding recovered from filtered unit rows by least squares
 //   mirror          (Ohnesorge eqs. 3a/3b, RTK's weights); thresholds of
 //                   test/rtkrampfiltertest.cxx:121-134 pass with it, fail without
