@@ -201,6 +201,20 @@ int rtkhip_column_filter(const int proj_dim[3], int ypad, const double * kernel,
 int rtkhip_cg_step(size_t n, float * dev_x, float * dev_r, float * dev_p, const float * dev_ap,
                    double * dev_state, void * stream);
 
+/* rtkhip_cg_step without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * reconstruction loops, harnesses that forbid hipMalloc).  Same arguments, kernels,
+ * grid and results (bit for bit) as rtkhip_cg_step; the FP64 partial sums of its two
+ * reductions live in a static 8 KiB device array (1024 doubles) instead of a buffer
+ * allocated by the library.  The kernels stride over the vectors with at most 1024
+ * workgroups, so every n fits; a grid above the array's size is refused with
+ * RTKHIP_ERR_UNSUPPORTED.  The array's device address is looked up on the first call
+ * on each device; each device keeps its copy until the process ends.  All calls on one
+ * device share that copy: calls that overlap on one device (other streams, other host
+ * threads) race and must be serialised by the caller. */
+int rtkhip_cg_step_noalloc(size_t n, float * dev_x, float * dev_r, float * dev_p, const float * dev_ap,
+                           double * dev_state, void * stream);
+
 /* R = B - AX; P = R; dev_state[0] = ||R||^2 (FP64).  dev_r may alias dev_ax. */
 int rtkhip_cg_init(size_t n, const float * dev_b, const float * dev_ax, float * dev_r, float * dev_p,
                    double * dev_state, void * stream);

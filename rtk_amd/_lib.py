@@ -63,6 +63,7 @@ SIGNATURES = [
     ("rtkhip_ramp_filter", I, [PI3, I, PD, I, PD, PD, PD, PD, P, P, I, P]),
     ("rtkhip_column_filter", I, [PI3, I, PD, P, P, I, P]),
     ("rtkhip_cg_step", I, [ctypes.c_size_t, P, P, P, P, P, P]),
+    ("rtkhip_cg_step_noalloc", I, [ctypes.c_size_t, P, P, P, P, P, P]),
     ("rtkhip_cg_init", I, [ctypes.c_size_t, P, P, P, P, P, P]),
     ("rtkhip_dot", I, [ctypes.c_size_t, P, P, P, P]),
     ("rtkhip_multiply", I, [ctypes.c_size_t, P, P, P, P]),

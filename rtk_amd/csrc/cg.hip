@@ -12,6 +12,7 @@
 
 #include <cfloat>
 #include <map>
+#include <mutex>
 
 namespace rtkhip
 {
@@ -143,6 +144,62 @@ Partials(hipStream_t s)
 } // namespace
 } // namespace rtkhip
 
+// Partial sums of rtkhip_cg_step_noalloc: one FP64 per workgroup of the reductions,
+// at most kMaxBlocks of them (8 KiB).  Each device that runs the entry point holds its
+// own copy until the process ends.  Host code never passes this name to a kernel: it
+// takes the device address from hipGetSymbolAddress (NoAllocScratch below).
+static __device__ __align__(256) double scratch[rtkhip::kMaxBlocks];
+
+namespace rtkhip
+{
+namespace
+{
+
+constexpr int kMaxDevices = 64;
+
+// Device address of `scratch` on the current device, looked up once per device with
+// that device current and kept.
+double *
+NoAllocScratch()
+{
+  static std::mutex lock;
+  static double *   ptr[kMaxDevices] = {};
+  int               dev = 0;
+  RTKHIP_CHECK(hipGetDevice(&dev));
+  RTKHIP_REQUIRE(dev >= 0 && dev < kMaxDevices, RTKHIP_ERR_UNSUPPORTED, "cg_step_noalloc: device index above 63");
+  std::lock_guard<std::mutex> g(lock);
+  if (!ptr[dev])
+  {
+    void * p = nullptr;
+    RTKHIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    RTKHIP_REQUIRE(p != nullptr, RTKHIP_ERR_HIP, "cg_step_noalloc: no device address for the scratch array");
+    ptr[dev] = static_cast<double *>(p);
+  }
+  return ptr[dev];
+}
+
+// The five launches of one CG step with the partial sums in `part` (at least Blocks(n)
+// doubles).
+void
+CgStepLaunches(size_t n, float * x, float * r, float * p, const float * ap, double * state, double * part,
+               hipStream_t s)
+{
+  const int g = Blocks(n);
+  hipLaunchKernelGGL(dot_kernel, dim3(g), dim3(kThreads), 0, s, n, p, ap, part);
+  RTKHIP_CHECK_LAUNCH();
+  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kThreads), 0, s, part, g, state, 1);
+  RTKHIP_CHECK_LAUNCH();
+  hipLaunchKernelGGL(cg_xr_kernel, dim3(g), dim3(kThreads), 0, s, n, x, r, p, ap, state, part);
+  RTKHIP_CHECK_LAUNCH();
+  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kThreads), 0, s, part, g, state, 2);
+  RTKHIP_CHECK_LAUNCH();
+  hipLaunchKernelGGL(cg_p_kernel, dim3(g), dim3(kThreads), 0, s, n, r, p, state);
+  RTKHIP_CHECK_LAUNCH();
+}
+
+} // namespace
+} // namespace rtkhip
+
 extern "C" int
 rtkhip_dot(size_t n, const float * dev_a, const float * dev_b, double * dev_result, void * stream)
 {
@@ -189,17 +246,25 @@ rtkhip_cg_step(size_t n, float * dev_x, float * dev_r, float * dev_p, const floa
                    "cg_step: null argument");
     RequireDevice();
     hipStream_t s = Stream(stream);
-    double *    part = Partials(s);
-    const int   g = Blocks(n);
-    hipLaunchKernelGGL(dot_kernel, dim3(g), dim3(kThreads), 0, s, n, dev_p, dev_ap, part);
-    RTKHIP_CHECK_LAUNCH();
-    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kThreads), 0, s, part, g, dev_state, 1);
-    RTKHIP_CHECK_LAUNCH();
-    hipLaunchKernelGGL(cg_xr_kernel, dim3(g), dim3(kThreads), 0, s, n, dev_x, dev_r, dev_p, dev_ap, dev_state, part);
-    RTKHIP_CHECK_LAUNCH();
-    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kThreads), 0, s, part, g, dev_state, 2);
-    RTKHIP_CHECK_LAUNCH();
-    hipLaunchKernelGGL(cg_p_kernel, dim3(g), dim3(kThreads), 0, s, n, dev_r, dev_p, dev_state);
-    RTKHIP_CHECK_LAUNCH();
+    CgStepLaunches(n, dev_x, dev_r, dev_p, dev_ap, dev_state, Partials(s), s);
+  });
+}
+
+extern "C" int
+rtkhip_cg_step_noalloc(size_t n, float * dev_x, float * dev_r, float * dev_p, const float * dev_ap,
+                       double * dev_state, void * stream)
+{
+  using namespace rtkhip;
+  return Guard([&] {
+    RTKHIP_REQUIRE(dev_x && dev_r && dev_p && dev_ap && dev_state, RTKHIP_ERR_INVALID_ARGUMENT,
+                   "cg_step_noalloc: null argument");
+    // The reductions keep one partial sum per workgroup and their grid strides over the
+    // vectors with at most kMaxBlocks workgroups, so every n fits the scratch array; this
+    // check guards that invariant.
+    RTKHIP_REQUIRE(Blocks(n) <= kMaxBlocks, RTKHIP_ERR_UNSUPPORTED,
+                   "cg_step_noalloc: " + std::to_string(Blocks(n)) + " partial sums exceed the scratch array's " +
+                     std::to_string(kMaxBlocks));
+    hipStream_t s = Stream(stream);
+    CgStepLaunches(n, dev_x, dev_r, dev_p, dev_ap, dev_state, NoAllocScratch(), s);
   });
 }

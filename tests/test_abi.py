@@ -44,6 +44,7 @@ def test_null_arguments_are_statuses_not_crashes():
     assert L.rtkhip_joseph_forward_project(None, None, None, None, None, 0., 1., None, None, None, None, None) == 1
     assert L.rtkhip_ramp_filter(None, 8, None, 0, None, None, None, None, None, None, 0, None) == 1
     assert L.rtkhip_cg_step(0, None, None, None, None, None, None) == 1
+    assert L.rtkhip_cg_step_noalloc(0, None, None, None, None, None, None) == 1
     m, mp = _lib.dbl(np.zeros(12))
     # unknown dtype is rejected before any device call
     assert L.rtkhip_fdk_backproject(pd, pd, mp, None, ctypes.c_void_p(8), ctypes.c_void_p(8), ctypes.c_void_p(8), 7,
