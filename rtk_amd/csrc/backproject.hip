@@ -48,20 +48,9 @@ struct alignas(16) ViewCoef
 // and that use the OPTIMIZED border rule (RTK's OptimizedBackprojectionY case).
 constexpr int kTileU = 64, kTileV = 48, kTileStride = 65; // staged detector footprint, pixels
 constexpr int kStagePasses = (kTileV + 15) / 16;           // 16 rows per staging pass
+constexpr int kPlaceViews = 64;                            // footprints placed per chunk, one per lane
 // 48 rows hold a 16-voxel column's taps up to 2.6 detector rows per voxel (the
 // 512^3 <- 1024^2 FDK reaches 2.34); each extra staging pass costs ~1 VALU per update
-
-// Lane exchange inside quads of lanes (DPP quad permutations, no LDS traffic).
-__device__ __forceinline__ float
-QuadXor1(float v)
-{
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true)); // [1,0,3,2]
-}
-__device__ __forceinline__ float
-QuadXor2(float v)
-{
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true)); // [2,3,0,1]
-}
 
 // Detector rectangle [u0, u0 + W) x [v0, v0 + H) holding every tap of the voxel block
 // [xlo, xhi] x [ylo, yhi] x [zlo, zhi] (centred coordinates) for view c.  u, v are
@@ -70,56 +59,41 @@ QuadXor2(float v)
 // backprojector's) the block's corners bound them.  One pixel of margin covers
 // rounding (including the 1-ulp reciprocal).  W = 0: no tap can pass the border rule
 // (skip the view); W < 0: too large, or w changes sign (read global memory for this
-// view).  Called by a whole wavefront: the four lanes of each quad evaluate the four
-// (x, z) corners (lane & 3) and combine them, so every lane returns the rectangle.
-__device__ __forceinline__ void
-FootprintQuad(const ViewCoef & c, float xlo, float xhi, float ylo, float yhi, float zlo, float zhi, int nu, int nv,
-              int lane, int & u0, int & v0, int & W, int & H)
+// view).  One lane evaluates the four (x, z) corners in turn, so a wavefront places
+// the footprints of 64 views at once.
+__device__ __forceinline__ int4
+FootprintLane(const ViewCoef & c, float xlo, float xhi, float ylo, float yhi, float zlo, float zhi, int nu, int nv)
 {
-  const float x = (lane & 1) ? xhi : xlo, z = (lane & 2) ? zhi : zlo;
-  const float w = c.c[8] + c.c[9] * x + c.c[11] * z;
-  const float rw = __builtin_amdgcn_rcpf(w);
-  const float u = (c.c[0] + c.c[1] * x + c.c[3] * z) * rw;
-  const float vl = (c.c[4] + c.c[5] * x + c.c[7] * z + c.c[6] * ylo) * rw;
-  const float vh = (c.c[4] + c.c[5] * x + c.c[7] * z + c.c[6] * yhi) * rw;
-  float       umin = u, umax = u, vmin = fminf(vl, vh), vmax = fmaxf(vl, vh);
-  // +1 / -1 where w > 0 / w < 0 at this corner, 0 otherwise: the block keeps one
-  // sign iff the quad's minimum and maximum are equal and non-zero
-  float sgn = (w > 0.f) ? 1.f : (w < 0.f ? -1.f : 0.f), smin = sgn, smax = sgn;
-  umin = fminf(umin, QuadXor1(umin));
-  umax = fmaxf(umax, QuadXor1(umax));
-  vmin = fminf(vmin, QuadXor1(vmin));
-  vmax = fmaxf(vmax, QuadXor1(vmax));
-  smin = fminf(smin, QuadXor1(smin));
-  smax = fmaxf(smax, QuadXor1(smax));
-  umin = fminf(umin, QuadXor2(umin));
-  umax = fmaxf(umax, QuadXor2(umax));
-  vmin = fminf(vmin, QuadXor2(vmin));
-  vmax = fmaxf(vmax, QuadXor2(vmax));
-  smin = fminf(smin, QuadXor2(smin));
-  smax = fmaxf(smax, QuadXor2(smax));
-  u0 = v0 = 0;
-  H = 0;
+  float umin = FLT_MAX, umax = -FLT_MAX, vmin = FLT_MAX, vmax = -FLT_MAX, smin = 1.f, smax = -1.f;
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+  {
+    const float x = (q & 1) ? xhi : xlo, z = (q & 2) ? zhi : zlo;
+    const float w = c.c[8] + c.c[9] * x + c.c[11] * z;
+    const float rw = __builtin_amdgcn_rcpf(w);
+    const float u = (c.c[0] + c.c[1] * x + c.c[3] * z) * rw;
+    const float vl = (c.c[4] + c.c[5] * x + c.c[7] * z + c.c[6] * ylo) * rw;
+    const float vh = (c.c[4] + c.c[5] * x + c.c[7] * z + c.c[6] * yhi) * rw;
+    const float sgn = (w > 0.f) ? 1.f : (w < 0.f ? -1.f : 0.f);
+    umin = fminf(umin, u);
+    umax = fmaxf(umax, u);
+    vmin = fminf(vmin, fminf(vl, vh));
+    vmax = fmaxf(vmax, fmaxf(vl, vh));
+    smin = fminf(smin, sgn);
+    smax = fmaxf(smax, sgn);
+  }
   if (!(smin == smax && smin != 0.f) || !(umin <= umax) || !(vmin <= vmax) || !(umax - umin < 1e6f) ||
       !(vmax - vmin < 1e6f))
-  {
-    W = -1;
-    return;
-  }
+    return make_int4(0, 0, -1, 0);
   const float fuLo = floorf(fmaxf(umin, -2.f)), fuHi = floorf(fminf(umax, float(nu)));
   const float fvLo = floorf(fmaxf(vmin, -2.f)), fvHi = floorf(fminf(vmax, float(nv)));
   if (fuHi < 0.f || fuLo > float(nu - 2) || fvHi < 0.f || fvLo > float(nv - 2))
-  {
-    W = 0;
-    return;
-  }
-  u0 = max(int(fuLo) - 1, 0) & ~3;
-  v0 = max(int(fvLo) - 1, 0);
+    return make_int4(0, 0, 0, 0);
+  const int u0 = max(int(fuLo) - 1, 0) & ~3;
+  const int v0 = max(int(fvLo) - 1, 0);
   const int u1 = min(int(fuHi) + 2, nu - 1), v1 = min(int(fvHi) + 2, nv - 1);
-  W = u1 - u0 + 1;
-  H = v1 - v0 + 1;
-  if (W > kTileU || H > kTileV)
-    W = -1;
+  const int W = u1 - u0 + 1, H = v1 - v0 + 1;
+  return make_int4(u0, v0, (W > kTileU || H > kTileV) ? -1 : W, H);
 }
 
 // Each thread accumulates a column of NJ voxels along y; a workgroup is 16 (x) x 16
@@ -163,7 +137,7 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
                                                      float ci, float cj, float ck)
 {
   __shared__ float stage[2][kTileV * kTileStride];
-  __shared__ int   rects[2][4]; // footprints of the next two views (u0, v0, W, H), by thread 0
+  __shared__ int4  rects[2][kPlaceViews]; // footprints (u0, v0, W, H) of two chunks of views
   const int        t = threadIdx.x;
   const int        bx0 = blockIdx.x * 16, bz0 = blockIdx.y * 16, j0 = blockIdx.z * NJ;
   const int        i = bx0 + (t & 7) + 8 * ((t >> 6) & 1);
@@ -242,28 +216,27 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
       }
     }
   };
-  // one wavefront places the footprint of view q two views ahead, in rects[q & 1]
-  // (wave q & 3, so the four waves share the work); the slot it overwrites was read
-  // before the barrier that precedes the write
+  // Wavefront 0 places the footprints of views [64 c, 64 c + 64) in rects[c & 1], one
+  // view per lane: chunk 0 before the loop, chunk c + 1 at view 64 c.  The buffer it
+  // overwrites held chunk c - 1, whose last footprint was read before the barrier
+  // that ended view 64 c - 1, and chunk c + 1 is first read at view 64 c + 63.
   const int wave = t >> 6;
-  auto      publish = [&](int q) {
-    int r[4];
-    FootprintQuad(views[q], xlo, xhi, ylo, yhi, zlo, zhi, nu, nv, t & 63, r[0], r[1], r[2], r[3]);
-    if ((t & 63) == 0)
-      for (int e = 0; e < 4; e++)
-        rects[q & 1][e] = r[e];
+  auto      place = [&](int chunk) {
+    const int q = chunk * kPlaceViews + (t & 63);
+    if (q < nviews)
+      rects[chunk & 1][t & 63] = FootprintLane(views[q], xlo, xhi, ylo, yhi, zlo, zhi, nu, nv);
+  };
+  auto rectOf = [&](int q, int & u0, int & v0, int & W, int & H) {
+    const int4 r = rects[(q / kPlaceViews) & 1][q % kPlaceViews];
+    u0 = __builtin_amdgcn_readfirstlane(r.x);
+    v0 = __builtin_amdgcn_readfirstlane(r.y);
+    W = __builtin_amdgcn_readfirstlane(r.z);
+    H = __builtin_amdgcn_readfirstlane(r.w);
   };
   if (wave == 0)
-  {
-    publish(0);
-    if (nviews > 1)
-      publish(1);
-  }
+    place(0);
   __syncthreads();
-  cu0 = rects[0][0];
-  cv0 = rects[0][1];
-  cW = rects[0][2];
-  cH = rects[0][3];
+  rectOf(0, cu0, cv0, cW, cH);
   load(0, cu0, cv0, cW, cH);
   store(0, cW, cH);
   __syncthreads();
@@ -273,14 +246,11 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
     int       nu0 = 0, nv0 = 0, nW = 0, nH = 0;
     if (p + 1 < nviews)
     {
-      nu0 = rects[(p + 1) & 1][0];
-      nv0 = rects[(p + 1) & 1][1];
-      nW = rects[(p + 1) & 1][2];
-      nH = rects[(p + 1) & 1][3];
+      rectOf(p + 1, nu0, nv0, nW, nH);
       load(p + 1, nu0, nv0, nW, nH);
     }
-    if (wave == (p & 3) && p + 2 < nviews)
-      publish(p + 2);
+    if (wave == 0 && p % kPlaceViews == 0 && p + kPlaceViews < nviews)
+      place(p / kPlaceViews + 1);
     // per-view limits of the fast path (wave-uniform)
     const bool  viewFast = jn == NJ && cW > 0;
     const float fvLo = float(cv0), fvHi = float(min(nv - 1, cv0 + cH - 1));
