@@ -236,8 +236,9 @@ def weighting_measurement(rtk, np, torch, proj, reps=3):
 
 def c3_projections(rtk, np, torch, g, nu, n):
     """C3's noisy stack: FP64 analytic line integrals of the 20-ellipsoid phantom (float32),
-    Poisson counts at I0 = 1e4 drawn by numpy on the host (the draw the double-precision
-    oracle's fixture used), converted back to line integrals."""
+    Poisson counts at I0 = 1e4 drawn per pixel by inversion of numpy uniforms on the host
+    (the draw the double-precision oracle's fixture used; the HIP line integrals equal the
+    oracle's to FP32 rounding), converted back to line integrals."""
     q = rtk.phantoms.random_ellipsoids(C3_PHANTOM_SEED, n=20, half_fov=256 * 1.0 / 2)
     det0 = -(nu - 1) * 0.8 / 2
     clean = rtk.project_quadrics(g, rtk.ConstantImageSource((nu, nu, n), (0.8, 0.8, 1.), (det0, det0, 0.)), q)
@@ -262,6 +263,20 @@ def c3_error_vs_oracle(np, torch, x, res, fx):
             "residual_history_max_rel_dev": float(np.max(np.abs(r / fx["residual"][:len(r)] - 1.)))}
 
 
+def c3_float_oracle_error(np, fx):
+    """The float oracle's own 30-iteration result (RTK's CPU filters in float, CG scalars
+    in float) against the double one, from the fixture: the FP32 scale of this problem."""
+    if "slice_y_f32" not in fx:
+        return None
+
+    def rl2(u, v):
+        return float(np.linalg.norm(np.asarray(u, np.float64) - v) / np.linalg.norm(v))
+    return {"rel_l2_slice_y": rl2(fx["slice_y_f32"], fx["slice_y"]),
+            "rel_l2_slice_z": rl2(fx["slice_z_f32"], fx["slice_z"]),
+            "rel_l2_block_means": rl2(fx["block_means_f32"], fx["block_means"]),
+            "residual_history_max_rel_dev": float(np.max(np.abs(fx["residual_f32"] / fx["residual"] - 1.)))}
+
+
 def cg_measurement(rtk, np, torch, max_iter, tol):
     """Plain CG on 256^3 @1 mm from 360 views of 512^2 @0.8 mm with Poisson noise
     (I0 = 1e4): 30 iterations from zero with RTK's default pair (Joseph FP + voxel
@@ -274,7 +289,10 @@ def cg_measurement(rtk, np, torch, max_iter, tol):
         g.AddProjection(1000., 1536., i * 360. / n)
     proj, counts_sum = c3_projections(rtk, np, torch, g, nu, n)
     fx = dict(np.load(C3_FIXTURE)) if os.path.exists(C3_FIXTURE) else None
-    if fx is not None and int(fx["counts_sum"]) != counts_sum:
+    # the fixture's input: the same counts up to the few pixels a rounding step moves
+    # across a count boundary
+    matches = fx is not None and abs(int(fx["counts_sum"]) - counts_sum) <= 1e-9 * int(fx["counts_sum"])
+    if fx is not None and not matches:
         log(f"cg: Poisson counts sum {counts_sum} differs from the oracle fixture's {int(fx['counts_sum'])}")
     vol = rtk.ConstantImageSource((nx, nx, nx), (1., 1., 1.), (-(nx - 1) / 2.,) * 3)
     out = {}
@@ -305,7 +323,7 @@ def cg_measurement(rtk, np, torch, max_iter, tol):
             out[name]["reached"] = bool(res[-1] <= t)
         x = cg.GetOutput().tensor
         if name.startswith("fixed_30"):
-            if fx is not None:
+            if matches:
                 out[name]["vs_fp64_oracle"] = c3_error_vs_oracle(np, torch, x, res, fx)
             if name == "fixed_30_voxel_bp":
                 ref = x.double()
@@ -317,7 +335,8 @@ def cg_measurement(rtk, np, torch, max_iter, tol):
     return {"workload": "CG 256^3 @1 mm <- 360 x 512^2 @0.8 mm, Poisson I0=1e4, zero start; FP32/FP16/BF16 stacks "
                         "read by the backprojector (FP32 arithmetic, FP64 CG scalars)",
             "target_rel_residual": tol, "max_iterations": max_iter,
-            "input_matches_oracle_fixture": None if fx is None else bool(int(fx["counts_sum"]) == counts_sum),
+            "input_matches_oracle_fixture": None if fx is None else bool(matches),
+            "float_oracle_vs_fp64_oracle": c3_float_oracle_error(np, fx) if fx is not None else None,
             **out}
 
 

@@ -95,11 +95,45 @@ def random_ellipsoids(seed, n=20, half_fov=128.0, base=0.02, base_radius=0.9, ce
     return np.stack(rows)
 
 
-def poisson_line_integrals(p, i0, seed, return_counts=False):
-    """Noisy line integrals: counts ~ Poisson(I0 exp(-p)), p_hat = -ln(max(c, 1) / I0)
-    (numpy's generator, so the same float32 p and seed give the same draw on any host).
-    return_counts: also return the counts."""
-    rng = np.random.default_rng(seed)
-    c = rng.poisson(i0 * np.exp(-np.asarray(p, dtype=np.float64)))
+def _poisson_inverse(lam, u):
+    """Smallest k with P(Poisson(lam) <= k) >= u, element-wise: a Cornish-Fisher start
+    (lam + sqrt(lam) z + (z^2 - 1) / 6, z = Phi^-1(u)) corrected one count at a time
+    against the exact CDF, P(X <= k) = Q(k + 1, lam) (scipy.special.pdtr)."""
+    from scipy import special
+    z = special.ndtri(u)
+    k = np.nan_to_num(np.floor(lam + np.sqrt(lam) * z + (z * z - 1.) / 6.), nan=0., posinf=0., neginf=0.)
+    np.maximum(k, 0., out=k)
+    for _ in range(200):
+        low = special.pdtr(k, lam) < u
+        k[low] += 1.
+        high = (k > 0.) & (special.pdtr(k - 1., lam) >= u)
+        k[high] -= 1.
+        if not low.any() and not high.any():
+            return k.astype(np.int64)
+    raise RuntimeError("Poisson inversion did not converge")
+
+
+def poisson_line_integrals(p, i0, seed, return_counts=False, threads=16):
+    """Noisy line integrals: counts ~ Poisson(I0 exp(-p)), p_hat = -ln(max(c, 1) / I0).
+
+    The counts are drawn by inversion, one uniform variate per pixel from numpy's
+    generator (`seed`), so a pixel's count depends only on its own line integral and
+    its own variate: the same float32 p and seed give the same draw on any host, and
+    two stacks that differ in a few pixels by a rounding step draw the same counts
+    everywhere else (a sequential sampler would change every later draw).
+    return_counts: also return the counts.  `threads` workers share the inversion."""
+    from concurrent.futures import ThreadPoolExecutor
+    lam = i0 * np.exp(-np.asarray(p, dtype=np.float64))
+    u = np.random.default_rng(seed).random(lam.shape)
+    flat_l, flat_u = lam.ravel(), u.ravel()
+    c = np.empty(flat_l.shape, np.int64)
+    chunk = 1 << 20
+
+    def run(a):
+        c[a:a + chunk] = _poisson_inverse(flat_l[a:a + chunk], flat_u[a:a + chunk])
+
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        list(ex.map(run, range(0, flat_l.size, chunk)))
+    c = c.reshape(lam.shape)
     out = (-np.log(np.maximum(c, 1) / i0)).astype(np.float32)
     return (out, c) if return_counts else out

@@ -4,7 +4,8 @@ Workload: a 256^3 volume at 1 mm (centred) reconstructed from 360 views of 512 x
 pixels at 0.8 mm (SID 1000 mm, SDD 1536 mm, 360 degrees), the projections being the
 FP64 analytic line integrals of the seeded 20-ellipsoid phantom (seed PHANTOM_SEED,
 rounded to float32 as the HIP phantom projector stores them), with Poisson noise at
-I0 = 1e4 drawn by numpy (seed NOISE_SEED) and converted back to line integrals.
+I0 = 1e4 drawn per pixel by inversion of numpy uniforms (seed NOISE_SEED,
+rtk_amd.phantoms.poisson_line_integrals) and converted back to line integrals.
 Thirty iterations of CG from zero with RTK's default projector pair (Joseph forward,
 voxel-based back projection; include/rtkConjugateGradientConeBeamReconstructionFilter.hxx:195-305),
 scalars in double.
@@ -18,8 +19,14 @@ Stored (the volume itself is 64 MB, so only what the GPU test compares):
   x_norm, x_sum  ||X_30|| and sum(X_30)
   counts_sum     sum of the Poisson counts (checks that the GPU test drew the same input)
 
-The oracle needs about an hour on 8 cores.  Usage (repository root):
-  python tests/golden/make_cg_c3.py [--threads 8]
+With --float, the same 30 iterations run through the float oracle (RTK's float
+filters restated, CG scalars in float as in include/rtkConjugateGradientImageFilter.hxx)
+on the same input, and its residual history, planes and block means are added to the
+fixture under the same names with the suffix _f32: how far an FP32 implementation of
+RTK's own algorithm lies from the double one on this problem.
+
+Each run needs about an hour on 8 cores.  Usage (repository root):
+  python tests/golden/make_cg_c3.py [--threads 8] [--float]
 """
 import argparse
 import os
@@ -56,33 +63,47 @@ def quadrics():
     return phantoms.random_ellipsoids(PHANTOM_SEED, n=20, half_fov=NX / 2.)
 
 
-def noisy(clean32):
-    """Poisson counts and noisy line integrals from the float32 clean stack: the GPU
-    test applies this same function to the HIP phantom projector's output."""
-    rng = np.random.default_rng(NOISE_SEED)
-    counts = rng.poisson(I0 * np.exp(-clean32.astype(np.float64)))
-    p = (-np.log(np.maximum(counts, 1) / I0)).astype(np.float32)
+def noisy(clean32, threads=8):
+    """Poisson counts and noisy line integrals from the float32 clean stack, drawn by
+    rtk_amd.phantoms.poisson_line_integrals (per-pixel inversion): the GPU test and the
+    benchmark apply the same function to the HIP phantom projector's output, which
+    equals this one's to FP32 rounding."""
+    p, counts = phantoms.poisson_line_integrals(clean32, I0, NOISE_SEED, return_counts=True, threads=threads)
     return counts, p
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--threads", type=int, default=8)
+    ap.add_argument("--float", action="store_true", help="add the float oracle's results to the fixture")
     args = ap.parse_args()
     s, v = stack_image(), volume_image()
     st = rtkref.image(s["size"], s["origin"], s["spacing"])
     ov = rtkref.image(v["size"], v["origin"], v["spacing"])
     t0 = time.time()
     clean = rtkref.project_quadrics(rows(), st, np.zeros((NVIEWS, NU, NU)), quadrics(), threads=args.threads)
-    counts, p = noisy(clean.astype(np.float32))
+    counts, p = noisy(clean.astype(np.float32), args.threads)
     del clean
     print(f"projections {time.time() - t0:.0f} s, counts sum {int(counts.sum())}", flush=True)
+    path = os.path.join(HERE, "cg_c3.npz")
+    if args.float:
+        fx = dict(np.load(path))
+        assert int(counts.sum()) == int(fx["counts_sum"])
+        x, res2 = rtkref.conjugate_gradient(rows(), ov, np.zeros((NX,) * 3, np.float32), st, p, NITER,
+                                            threads=args.threads)
+        print(f"float CG {time.time() - t0:.0f} s", flush=True)
+        xd = x.astype(np.float64)
+        fx.update(residual_f32=np.sqrt(res2 / res2[0]),
+                  block_means_f32=xd.reshape(32, 8, 32, 8, 32, 8).mean(axis=(1, 3, 5)).astype(np.float32),
+                  slice_y_f32=x[:, NX // 2, :].copy(), slice_z_f32=x[NX // 2].copy())
+        np.savez_compressed(path, **fx)
+        print(f"{path}: {os.path.getsize(path)} bytes", flush=True)
+        return
     x, res2 = rtkref.conjugate_gradient(rows(), ov, np.zeros((NX,) * 3), st, p.astype(np.float64), NITER,
                                         threads=args.threads)
     print(f"CG {time.time() - t0:.0f} s", flush=True)
     res = np.sqrt(res2 / res2[0])
     bm = x.reshape(32, 8, 32, 8, 32, 8).mean(axis=(1, 3, 5))
-    path = os.path.join(HERE, "cg_c3.npz")
     np.savez_compressed(path, residual=res, b_norm=np.sqrt(res2[0]), block_means=bm.astype(np.float32),
                         slice_y=x[:, NX // 2, :].astype(np.float32), slice_z=x[NX // 2].astype(np.float32),
                         x_norm=np.linalg.norm(x), x_sum=x.sum(), counts_sum=np.int64(counts.sum()),

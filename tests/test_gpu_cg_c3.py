@@ -2,7 +2,7 @@
 
 Workload: 256^3 @1 mm from 360 views of 512^2 @0.8 mm (SID 1000 mm, SDD 1536 mm),
 analytic projections of the seeded 20-ellipsoid phantom with Poisson noise at
-I0 = 1e4 drawn by numpy (the same draw tests/golden/make_cg_c3.py made), 30
+I0 = 1e4 drawn per pixel (the same draw tests/golden/make_cg_c3.py made), 30
 iterations from zero with RTK's default pair (Joseph forward, voxel-based back
 projection).  tests/golden/cg_c3.npz holds the oracle's residual history, central
 planes and 8^3 block means of X_30, computed in double.
@@ -14,7 +14,9 @@ Checks:
     of X_30, not the CG recursion) agrees with the recursion's last residual;
   * the first step is X_1 = alpha_0 B with alpha_0 = B.B / B.AB, in FP64;
   * FP16 and BF16 stacks: X_30's error against the oracle, within the bars stated at
-    test_c3_16bit_stacks_vs_fp64_oracle.
+    test_c3_16bit_stacks_vs_fp64_oracle;
+  * the float oracle's result (fixture keys *_f32: RTK's CPU filters in float, CG
+    scalars in float) lies far further from the double oracle than this path does.
 """
 import os
 
@@ -42,12 +44,15 @@ def c3(hip):
     q = phantoms.random_ellipsoids(int(fx["phantom_seed"]), n=20, half_fov=NX / 2.)
     det0 = -(NU - 1) * 0.8 / 2
     clean = hip.project_quadrics(g, hip.ConstantImageSource((NU, NU, NVIEWS), (0.8, 0.8, 1.), (det0, det0, 0.)), q)
+    # per-pixel Poisson draws: the HIP line integrals equal the oracle's to FP32
+    # rounding, so the counts equal the fixture's except where a rounding step moves a
+    # pixel across a count boundary (none or a few of the 9.4e7 pixels)
     p, counts = phantoms.poisson_line_integrals(clean.numpy(), float(fx["i0"]), int(fx["noise_seed"]),
                                                 return_counts=True)
-    # the HIP and oracle phantom projectors agree to FP64 rounding, so the float32 line
-    # integrals, and with them the Poisson draw, are the oracle's
-    assert int(counts.sum()) == int(fx["counts_sum"]), (int(counts.sum()), int(fx["counts_sum"]))
+    assert abs(int(counts.sum()) - int(fx["counts_sum"])) <= 1e-9 * int(fx["counts_sum"]), \
+        (int(counts.sum()), int(fx["counts_sum"]))
     proj = clean.like(torch.from_numpy(p).cuda())
+    del clean
     vol = hip.ConstantImageSource((NX,) * 3, (1., 1., 1.), (-(NX - 1) / 2.,) * 3)
     return g, proj, vol, fx
 
@@ -96,12 +101,25 @@ def test_c3_fp32_residual_history_and_volume_vs_oracle(hip, c3):
     x, res = _cg(hip, c3, 30)
     dev = np.abs(res / fx["residual"] - 1.)
     err = _vs_oracle(x, fx)
+    # the float oracle (RTK's CPU filters in float, CG scalars in float) against the
+    # double one on the same input: how far FP32 arithmetic of the reference algorithm
+    # itself lies from FP64 on this problem
+    ref32 = {k: rel_l2(fx[k + "_f32"], fx[k]) for k in ("slice_y", "slice_z", "block_means")}
     print(f"C3 FP32: residual history max rel dev {dev.max():.3g} (last {res[-1]:.6g} vs {fx['residual'][-1]:.6g}); "
-          f"X_30 vs oracle {err}")
-    # FP32 projector rounding (~1e-6 relative per operator application) grows through
-    # the 30 recursions of an unsymmetric operator; the bars leave a 3x margin
+          f"X_30 vs oracle {err}; float oracle vs double oracle {ref32}")
+    # The residual history and the 8^3 block means are smooth functionals: 1e-4.  The
+    # central planes hold the high-frequency noise that 30 iterations of CG have begun
+    # to fit, where the FP32 projectors' rounding is amplified: 1e-3 (measured 2-5e-4).
+    # The float oracle, whose CG scalars are float as in RTK, lies 4-5e-2 from the
+    # double one on these planes and 0.44 on the residual history: this path, with its
+    # FP64 scalars, must stay at least 20 times closer on each.
     assert dev.max() <= 1e-4
-    assert max(err.values()) <= 1e-4
+    assert err["block_means"] <= 1e-4
+    for k in ("slice_y", "slice_z"):
+        assert err[k] <= 1e-3, (k, err[k])
+    for k in ("slice_y", "slice_z", "block_means"):
+        assert 20. * err[k] <= ref32[k], (k, err[k], ref32[k])
+    assert 20. * dev.max() <= float(np.max(np.abs(fx["residual_f32"] / fx["residual"] - 1.)))
     # the recursion's residual against a direct FP64 evaluation of ||B - A X_30||
     b = _bp(hip, g, proj, vol).double()
     r = b - _apply_a(hip, g, proj, x).double()
@@ -123,16 +141,19 @@ def test_c3_first_step_is_alpha0_times_b(hip, c3):
     assert e <= 1e-5
 
 
-@pytest.mark.parametrize("dtype,bar", [("f16", 2e-3), ("bf16", 5e-2)])
-def test_c3_16bit_stacks_vs_fp64_oracle(hip, c3, dtype, bar):
+@pytest.mark.parametrize("dtype,unit", [("f16", 2. ** -11), ("bf16", 2. ** -8)])
+def test_c3_16bit_stacks_vs_fp64_oracle(hip, c3, dtype, unit):
     """The stacks the backprojector reads (the weighted data once, W FP(P) every
-    iteration) are rounded to FP16 (11 significant bits, relative rounding 2^-12) or
-    BF16 (8 bits, 2^-9).  CG from zero after 30 iterations amplifies a perturbation of
-    the data by the conditioning of the high-frequency components it has started to
-    fit, measured at about 5x to 10x the per-element rounding, so the bars are 2e-3
-    (FP16) and 5e-2 (BF16) relative L2 against the oracle."""
+    iteration) are rounded to FP16 or BF16, whose unit roundoff u is 2^-11 and 2^-8.
+    Thirty iterations of CG from zero carry that rounding into X_30 amplified on the
+    noise-dominated central planes (measured about 6 u for both formats) and averaged
+    out on 8^3 block means (about 0.2 u): the bars are 12 u on the planes and u on the
+    block means against the double-precision oracle."""
     _, _, _, fx = c3
     x, res = _cg(hip, c3, 30, dtype)
     err = _vs_oracle(x, fx)
-    print(f"C3 {dtype} stacks: X_30 vs oracle {err}, last residual {res[-1]:.6g} (oracle {fx['residual'][-1]:.6g})")
-    assert max(err.values()) <= bar
+    print(f"C3 {dtype} stacks: X_30 vs oracle {err} (in units of u: "
+          f"{ {k: round(v / unit, 2) for k, v in err.items()} }), last residual {res[-1]:.6g} "
+          f"(oracle {fx['residual'][-1]:.6g})")
+    assert err["slice_y"] <= 12. * unit and err["slice_z"] <= 12. * unit
+    assert err["block_means"] <= unit
