@@ -31,10 +31,12 @@
 
 #include <hip/amd_detail/amd_hip_unsafe_atomics.h>
 
+#include <array>
 #include <cfloat>
 #include <climits>
 #include <cmath>
 #include <map>
+#include <type_traits>
 #include <vector>
 
 namespace rtkhip
@@ -822,13 +824,746 @@ __global__ __launch_bounds__(256) void joseph_bp_tile_kernel(const JosephView * 
   }
 }
 
+// ---------------------------------------------------- gather backprojection --
+// The same backprojection computed voxel by voxel, without atomics, for geometries
+// whose detector columns are vertical lines in the volume (no in-plane or out-of-plane
+// tilt: a ray's horizontal position depends on its column only, its height on its row
+// only, and every ray of a column has the same dominant axis, x or z).
+//
+// A ray of column iu walks slices k along its dominant axis A; at slice k its in-slice
+// position is (X, y): X along the other horizontal axis, a Moebius function of iu, and
+// y, affine in the row jv.  Its Joseph splat adds s_k v |stepMM| T(X - h) T(y - j) to
+// voxel (k, j, h) of the slice (T(d) = max(0, 1 - |d|), the bilinear weight of a tap),
+// with s_k = 1 on middle steps and the partial weights wB / wE / wOne on its first and
+// last slices, whose positions are clamped into the box (BilinearSplatOnBorders).  So a
+// voxel receives from the rays passing within one voxel of it in its slice: about
+// 2 / (ray spacing) columns times as many rows.  A voxel whose three indices lie in
+// [2, N - 3] only ever receives middle steps: a first or last step lies on a slice face
+// (k = 0 or N - 1) or within half a voxel of a side face.  Those voxels need only the
+// ray values; the two outer layers (the "shell") also read each ray's first and last
+// slice and weights.  Rays whose dominant axis is not their column's (y-dominant rays
+// of steep cones, ties decided differently in FP64) and rays carrying non-finite values
+// are splatted by the per-ray path instead, so the result is RTK's for every ray.
+
+constexpr int kGTileU = 64, kGTileV = 64, kGStride = 65; // staged ray values: columns x rows
+constexpr int kGRows = 5;                                // candidate rows per voxel and column
+constexpr int kGNJ = 16;                                 // voxels per thread along y
+
+struct GatherView
+{
+  float  pu[4], pv[4], pw[4]; // centred volume index (x', y', z', 1) -> detector (u w, v w, w)
+  double y0, y1, sy;          // y' of row jv at depth t: (y0 + y1 jv)(1 - t) + sy t
+  int    hasX, hasZ;          // columns walking along x / along z exist in this view
+  int    pad[2];
+};
+
+struct GatherColumn
+{
+  double h0, h1; // in-slice horizontal coordinate X' = h0 + k' h1 at centred slice k'
+  double t0, t1; // ray parameter t = t0 + k' t1 (0 at the pixel, 1 at the source)
+};
+
+// Per-ray data of the gather: value * |stepMM| (0 for rays the gather must not take);
+// the same value beside the first/last slice (16 bits each) and their weights.
+// Views [p0, p0 + gridDim.y) of the call; gval/gend/gw and colAxis are indexed by the
+// view's position in the chunk.
+template <typename TP>
+__global__ __launch_bounds__(256) void joseph_gather_prep_kernel(
+  const JosephView * __restrict__ views, int p0, const signed char * __restrict__ colAxis, int nu, int nv,
+  const TP * __restrict__ proj, float * acc, float * accT, int nx, int ny, int nz, double spx, double spy,
+  double spz, double infClip, double supClip, float * __restrict__ gval, uint2 * __restrict__ gend,
+  float2 * __restrict__ gw)
+{
+  const int iu = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int jv = blockIdx.z * 16 + (threadIdx.x >> 4);
+  const int pl = blockIdx.y, p = p0 + pl;
+  if (iu >= nu || jv >= nv)
+    return;
+  const size_t pix = (size_t(pl) * nv + jv) * nu + iu;
+  const float  value = LoadPix(proj, (long(p) * nv + jv) * nu + iu);
+  Ray          R;
+  float        vn = 0.f;
+  unsigned     e = 0u;
+  float2       w = make_float2(0.f, 0.f);
+  if (value != 0.f && SetupRay<true>(views[p], iu, jv, p, nx, ny, nz, spx, spy, spz, infClip, supClip, R))
+  {
+    if (R.axis == int(colAxis[size_t(pl) * nu + iu]) && isfinite(value))
+    {
+      vn = float(double(value) * R.stepNorm);
+      e = unsigned(R.ns) | (unsigned(R.fs) << 16);
+      w = make_float2(float(R.fs == R.ns ? R.wOne : R.wB), float(R.wE));
+    }
+    else
+      SplatRay(R, value, acc, accT);
+  }
+  gval[pix] = vn;
+  gend[pix] = make_uint2(__float_as_uint(vn), e); // the value again, so the shell reads one 8-byte word
+  gw[pix] = w;
+}
+
+// Rectangle of rays [u0, u0 + W) x [v0, v0 + H) that can reach the voxel block
+// [xlo, xhi] x [ylo, yhi] x [zlo, zhi] (centred, already widened by one voxel): the
+// rays crossing the widened block, bounded by the central projection of its corners.
+// W <= 0 when the rectangle does not fit the tile (the block then reads its rays from
+// global memory for this view) or no ray reaches it.
+__device__ __forceinline__ int4
+GatherRect(const GatherView & V, float xlo, float xhi, float ylo, float yhi, float zlo, float zhi, int nu, int nv)
+{
+  float umin = FLT_MAX, umax = -FLT_MAX, vmin = FLT_MAX, vmax = -FLT_MAX, smin = 1.f, smax = -1.f;
+#pragma unroll
+  for (int q = 0; q < 8; q++)
+  {
+    const float x = (q & 1) ? xhi : xlo, y = (q & 2) ? yhi : ylo, z = (q & 4) ? zhi : zlo;
+    const float w = V.pw[3] + V.pw[0] * x + V.pw[1] * y + V.pw[2] * z;
+    const float rw = 1.f / w;
+    const float u = (V.pu[3] + V.pu[0] * x + V.pu[1] * y + V.pu[2] * z) * rw;
+    const float v = (V.pv[3] + V.pv[0] * x + V.pv[1] * y + V.pv[2] * z) * rw;
+    const float sgn = (w > 0.f) ? 1.f : (w < 0.f ? -1.f : 0.f);
+    umin = fminf(umin, u);
+    umax = fmaxf(umax, u);
+    vmin = fminf(vmin, v);
+    vmax = fmaxf(vmax, v);
+    smin = fminf(smin, sgn);
+    smax = fmaxf(smax, sgn);
+  }
+  if (!(smin == smax && smin != 0.f) || !(umin <= umax) || !(vmin <= vmax) || !(umax - umin < 1e6f) ||
+      !(vmax - vmin < 1e6f))
+    return make_int4(0, 0, -1, 0);
+  // columns outside the detector hold no rays; rows are staged with kGRows rows of
+  // margin on both sides, rows outside the detector as zeros, so that every row a
+  // voxel reads (up to kGRows from its first candidate) lies in the tile
+  const int u0 = max(int(floorf(fmaxf(umin, -8.f))) - 1, 0) & ~3;
+  const int u1 = min(int(floorf(fminf(umax, float(nu) + 8.f))) + 2, nu - 1);
+  const int v0 = int(floorf(fmaxf(vmin, -1e6f))) - 1 - kGRows;
+  const int v1 = int(floorf(fminf(vmax, 1e6f))) + 2 + kGRows;
+  if (u1 < u0 || v1 < 0 || v0 > nv - 1)
+    return make_int4(0, 0, 0, 0);
+  const int W = u1 - u0 + 1, H = v1 - v0 + 1;
+  return make_int4(u0, v0, (W > kGTileU || H > kGTileV) ? -1 : W, H);
+}
+
+// Step weight of slice k for a ray with first/last slices (ns, fs) packed in e and
+// their weights w: 0 outside [ns, fs], 1 inside, wB / wE / wOne on the end slices.
+__device__ __forceinline__ float
+StepWeight(int k, unsigned e, float2 w)
+{
+  const int ns = int(e & 0xffffu), fs = int(e >> 16);
+  if (k < min(ns, fs) || k > max(ns, fs))
+    return 0.f;
+  if (k == ns)
+    return w.x;
+  return k == fs ? w.y : 1.f;
+}
+
+// Thread = a column of kGNJ voxels along y at (x, z); workgroup = 16 x 16 columns.
+// Per view the block's rays are staged in LDS (double-buffered, one barrier per view,
+// the next view's loads in flight during the current view's arithmetic).  Columns
+// whose x and z lie in [2, N - 3] are computed here: their voxels with y in [2, N - 3]
+// receive middle steps only; the two outer layers along y (a choice uniform across the
+// workgroup) also read each candidate ray's first/last slices and weights from global
+// memory.  Columns of the two outer layers along x or z are joseph_gather_shell_kernel's.
+__global__ __launch_bounds__(256) void joseph_gather_kernel(
+  const GatherView * __restrict__ gviews, const GatherColumn * __restrict__ gcols,
+  const signed char * __restrict__ colAxis, int nviews, int nu, int nv, const float * __restrict__ gval,
+  const uint2 * __restrict__ gend, const float2 * __restrict__ gw, const int * __restrict__ blocks, float * vol,
+  int nx, int ny, int nz)
+{
+  __shared__ float        tile[2][kGTileV * kGStride];
+  __shared__ GatherColumn cols[2][kGTileU];
+  __shared__ signed char  caxis[2][kGTileU];
+  __shared__ int4         rects[2][64];
+  const int               t = threadIdx.x;
+  const int               blk = blocks[blockIdx.x]; // packed (bx, bz, by)
+  const int               bx0 = (blk & 0x3ff) * 16, bz0 = ((blk >> 10) & 0x3ff) * 16, j0 = (blk >> 20) * kGNJ;
+  const int               i = bx0 + (t & 15), k = bz0 + (t >> 4);
+  // columns of the two outer layers along x or z hold only shell voxels
+  const bool              live = i >= 2 && i <= nx - 3 && k >= 2 && k <= nz - 3;
+  const float             cx = 0.5f * float(nx - 1), cy = 0.5f * float(ny - 1), cz = 0.5f * float(nz - 1);
+  const float             xc = float(i) - cx, zc = float(k) - cz, j0c = float(j0) - cy;
+  // widened block (one voxel on every side), centred
+  const float xlo = float(bx0 - 1) - cx, xhi = float(min(bx0 + 16, nx)) - cx;
+  const float zlo = float(bz0 - 1) - cz, zhi = float(min(bz0 + 16, nz)) - cz;
+  const float ylo = float(j0 - 1) - cy, yhi = float(min(j0 + kGNJ, ny)) - cy;
+  float       acc[kGNJ];
+#pragma unroll
+  for (int jj = 0; jj < kGNJ; jj++)
+    acc[jj] = 0.f;
+  // the block holds voxels of the outer layers along y (rows 0, 1, ny - 2, ny - 1)
+  const bool yshell = j0 < 2 || j0 + kGNJ > ny - 2;
+  float      ash[4] = {0.f, 0.f, 0.f, 0.f};
+  const size_t viewRays = size_t(nu) * nv;
+
+  // staging of view p's rectangle: lane = 4 consecutive rays of one row
+  const int sc = 4 * (t & 15), sr = t >> 4; // 16 rows per pass, 4 passes
+  float     sreg[16];
+  auto      load = [&](int p, int4 r) {
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+      {
+        const int row = sr + 16 * q, c = sc + e, jv = r.y + row;
+        sreg[4 * q + e] = (r.z > 0 && row < r.w && c < r.z && jv >= 0 && jv < nv)
+                            ? gval[size_t(p) * viewRays + size_t(jv) * nu + (r.x + c)]
+                            : 0.f;
+      }
+  };
+  auto store = [&](int b, int p, int4 r) {
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        tile[b][(sr + 16 * q) * kGStride + sc + e] = sreg[4 * q + e];
+    if (t < kGTileU && r.z > 0 && t < r.z)
+    {
+      cols[b][t] = gcols[size_t(p) * nu + r.x + t];
+      caxis[b][t] = colAxis[size_t(p) * nu + r.x + t];
+    }
+  };
+  const int wave = t >> 6;
+  auto      place = [&](int chunk) {
+    const int q = chunk * 64 + (t & 63);
+    if (q < nviews)
+      rects[chunk & 1][t & 63] = GatherRect(gviews[q], xlo, xhi, ylo, yhi, zlo, zhi, nu, nv);
+  };
+  auto rectOf = [&](int q) {
+    const int4 r = rects[(q / 64) & 1][q % 64];
+    return make_int4(__builtin_amdgcn_readfirstlane(r.x), __builtin_amdgcn_readfirstlane(r.y),
+                     __builtin_amdgcn_readfirstlane(r.z), __builtin_amdgcn_readfirstlane(r.w));
+  };
+  if (wave == 0)
+    place(0);
+  __syncthreads();
+  int4 cur = rectOf(0);
+  load(0, cur);
+  store(0, 0, cur);
+  __syncthreads();
+
+  const double xd = double(xc), zd = double(zc);
+  for (int p = 0; p < nviews; p++)
+  {
+    const int b = p & 1;
+    int4      nxt = make_int4(0, 0, 0, 0);
+    if (p + 1 < nviews)
+    {
+      nxt = rectOf(p + 1);
+      load(p + 1, nxt);
+    }
+    if (wave == 0 && p % 64 == 0 && p + 64 < nviews)
+      place(p / 64 + 1);
+    if (live && cur.z != 0)
+    {
+      const GatherView & V = gviews[p];
+      const bool         staged = cur.z > 0;
+#pragma unroll 1
+      for (int ax = 0; ax < 2; ax++)
+      {
+        const int axis = ax == 0 ? 0 : 2;
+        if ((axis == 0 && !V.hasX) || (axis == 2 && !V.hasZ))
+          continue;
+        // slice coordinate k' and in-slice horizontal coordinate h' of this voxel column
+        const float  kc = axis == 0 ? xc : zc, hc = axis == 0 ? zc : xc;
+        const double kd = axis == 0 ? xd : zd;
+        // candidate columns: the rays whose X' lies within one voxel of h' at slice k'
+        const float xa = axis == 0 ? kc : hc - 1.f, za = axis == 0 ? hc - 1.f : kc;
+        const float xb = axis == 0 ? kc : hc + 1.f, zb = axis == 0 ? hc + 1.f : kc;
+        const float ua = (V.pu[3] + V.pu[0] * xa + V.pu[2] * za) * __builtin_amdgcn_rcpf(V.pw[3] + V.pw[0] * xa + V.pw[2] * za);
+        const float ub = (V.pu[3] + V.pu[0] * xb + V.pu[2] * zb) * __builtin_amdgcn_rcpf(V.pw[3] + V.pw[0] * xb + V.pw[2] * zb);
+        int         iuLo = max(int(ceilf(fminf(ua, ub) - 1e-3f)), 0);
+        int         iuHi = min(int(floorf(fmaxf(ua, ub) + 1e-3f)), nu - 1);
+        if (staged)
+        {
+          iuLo = max(iuLo, cur.x);
+          iuHi = min(iuHi, cur.x + cur.z - 1);
+        }
+        // the staged and the global-memory paths are separate loops (the view's choice is
+        // uniform), so the column tables are read with LDS or global loads, not flat ones
+        auto columns = [&](auto stagedTag) {
+          constexpr bool STAGED = decltype(stagedTag)::value;
+          for (int iu = iuLo; iu <= iuHi; iu++)
+          {
+            const int    lc = iu - cur.x;
+            GatherColumn C;
+            int          cax;
+            if constexpr (STAGED)
+            {
+              C = cols[b][lc];
+              cax = int(caxis[b][lc]);
+            }
+            else
+            {
+              C = gcols[size_t(p) * nu + iu];
+              cax = int(colAxis[size_t(p) * nu + iu]);
+            }
+            if (cax != axis)
+              continue;
+            const double X = C.h0 + kd * C.h1;
+            const double tt = C.t0 + kd * C.t1;
+            const double a = V.y0 + tt * (V.sy - V.y0);
+            const double bd = V.y1 * (1. - tt);
+            // reference row near the block's centre height; e: y' of that row relative to j0
+            const float bf = float(bd), rb = __builtin_amdgcn_rcpf(bf), sg = bf > 0.f ? 1.f : -1.f;
+            const int   jref = int(rintf(float(double(j0c) + 0.5 * (kGNJ - 1) - a) * rb));
+            const float e = float(a + bd * jref - double(j0c));
+            const float wh = fmaxf(0.f, 1.f - fabsf(float(X - double(hc))));
+            const int   rowBase = jref - cur.y; // LDS row of jref
+            const int   kidx = axis == 0 ? i : k;   // this voxel column's slice index
+#pragma unroll
+            for (int jj = 0; jj < kGNJ; jj++)
+            {
+              const float base = e - float(jj);
+              const int   d0 = int(ceilf((-sg - base) * rb)); // first row with |y' - j| < 1
+              const float y0 = fmaf(bf, float(d0), base);     // y' - j of that row
+              float       s = 0.f;
+              // voxels of the outer layers along y: below (uniform)
+              if (j0 + jj < 2 || j0 + jj > ny - 3)
+                continue;
+              if constexpr (STAGED)
+              {
+                const int     r0 = min(max(rowBase + d0, 0), kGTileV - kGRows);
+                const float * src = &tile[b][r0 * kGStride + lc];
+#pragma unroll
+                for (int r = 0; r < kGRows; r++)
+                  s = fmaf(fmaxf(0.f, 1.f - fabsf(fmaf(bf, float(r), y0))), src[r * kGStride], s);
+              }
+              else
+              {
+#pragma unroll
+                for (int r = 0; r < kGRows; r++)
+                {
+                  const int jv = jref + d0 + r;
+                  if (jv >= 0 && jv < nv)
+                    s = fmaf(fmaxf(0.f, 1.f - fabsf(fmaf(bf, float(r), y0))),
+                             gval[size_t(p) * viewRays + size_t(jv) * nu + iu], s);
+                }
+              }
+              acc[jj] = fmaf(wh, s, acc[jj]);
+            }
+            if (yshell)
+            {
+              // rows 0, 1, ny - 2, ny - 1 (those in this block, each once): the step weight
+              // of this slice from the ray's first/last slices; on those, the height clamped
+              // into the box (a candidate's horizontal position at this interior column's
+              // slice lies inside the box, so it needs no clamp)
+#pragma unroll
+              for (int js = 0; js < 4; js++)
+              {
+                const int jy = js < 2 ? js : ny - 4 + js, jj = jy - j0;
+                if (jj < 0 || jj >= kGNJ || jy >= ny || (js >= 2 && jy < 2))
+                  continue;
+                const float base = e - float(jj);
+                const int   d0 = int(ceilf((-sg - base) * rb));
+                const float y0 = fmaf(bf, float(d0), base);
+                const float yv = j0c + float(jj);
+                float       s = 0.f;
+#pragma unroll 1
+                for (int r = 0; r < kGRows; r++)
+                {
+                  const int jv = jref + d0 + r;
+                  if (jv < 0 || jv >= nv)
+                    continue;
+                  const size_t pix = size_t(p) * viewRays + size_t(jv) * nu + iu;
+                  const uint2  ve = gend[pix];
+                  const float  val = __uint_as_float(ve.x);
+                  const int    ns = int(ve.y & 0xffffu), fs = int(ve.y >> 16);
+                  if (val == 0.f || kidx < min(ns, fs) || kidx > max(ns, fs))
+                    continue;
+                  const float yrel = fmaf(bf, float(r), y0);
+                  if (kidx == ns || kidx == fs)
+                  {
+                    const float2 w2 = gw[pix];
+                    const float  yc = fminf(fmaxf(yrel + yv, -cy), cy) - yv;
+                    s = fmaf((kidx == ns ? w2.x : w2.y) * fmaxf(0.f, 1.f - fabsf(yc)), val, s);
+                  }
+                  else
+                    s = fmaf(fmaxf(0.f, 1.f - fabsf(yrel)), val, s);
+                }
+                ash[js] = fmaf(wh, s, ash[js]);
+              }
+            }
+          }
+        };
+        if (staged)
+          columns(std::true_type{});
+        else
+          columns(std::false_type{});
+      }
+    }
+    if (p + 1 < nviews)
+      store(b ^ 1, p + 1, nxt);
+    __syncthreads();
+    cur = nxt;
+  }
+  if (live)
+  {
+#pragma unroll
+    for (int jj = 0; jj < kGNJ; jj++)
+      if (jj < ny - j0 && j0 + jj >= 2 && j0 + jj <= ny - 3)
+        vol[(size_t(k) * ny + (j0 + jj)) * nx + i] += acc[jj];
+    if (yshell)
+    {
+#pragma unroll
+      for (int js = 0; js < 4; js++)
+      {
+        const int jy = js < 2 ? js : ny - 4 + js, jj = jy - j0;
+        if (jj >= 0 && jj < kGNJ && jy < ny && !(js >= 2 && jy < 2))
+          vol[(size_t(k) * ny + jy) * nx + i] += ash[js];
+      }
+    }
+  }
+}
+
+// The voxels of the two outer layers along x and along z, one thread per voxel (voxel
+// indices from `shell`), reading the rays from global memory: these voxels also
+// receive first and last steps, so each candidate (ray, slice) takes its step weight
+// from the ray's first/last slices and, on those, its position clamped into the box.
+__global__ __launch_bounds__(256) void joseph_gather_shell_kernel(
+  const GatherView * __restrict__ gviews, const GatherColumn * __restrict__ gcols,
+  const signed char * __restrict__ colAxis, int nviews, int nu, int nv, const float * __restrict__ gval,
+  const uint2 * __restrict__ gend, const float2 * __restrict__ gw, const int * __restrict__ shell, int nshell,
+  float * vol, int nx, int ny, int nz)
+{
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nshell)
+    return;
+  const int    idx = shell[n];
+  const int    i = idx % nx, j = (idx / nx) % ny, k = idx / (nx * ny);
+  const float  cx = 0.5f * float(nx - 1), cy = 0.5f * float(ny - 1), cz = 0.5f * float(nz - 1);
+  const float  xc = float(i) - cx, zc = float(k) - cz, yv = float(j) - cy;
+  const double xd = double(xc), zd = double(zc);
+  const size_t viewRays = size_t(nu) * nv;
+  float        acc = 0.f;
+  for (int p = 0; p < nviews; p++)
+  {
+    const GatherView & V = gviews[p];
+    for (int ax = 0; ax < 2; ax++)
+    {
+      const int axis = ax == 0 ? 0 : 2;
+      if ((axis == 0 && !V.hasX) || (axis == 2 && !V.hasZ))
+        continue;
+      const float  kc = axis == 0 ? xc : zc, hc = axis == 0 ? zc : xc;
+      const double kd = axis == 0 ? xd : zd;
+      const int    kidx = axis == 0 ? i : k;
+      const float  hmax = axis == 0 ? cz : cx; // in-slice horizontal box [-hmax, hmax]
+      const float  xa = axis == 0 ? kc : hc - 1.f, za = axis == 0 ? hc - 1.f : kc;
+      const float  xb = axis == 0 ? kc : hc + 1.f, zb = axis == 0 ? hc + 1.f : kc;
+      const float  ua = (V.pu[3] + V.pu[0] * xa + V.pu[2] * za) * __builtin_amdgcn_rcpf(V.pw[3] + V.pw[0] * xa + V.pw[2] * za);
+      const float  ub = (V.pu[3] + V.pu[0] * xb + V.pu[2] * zb) * __builtin_amdgcn_rcpf(V.pw[3] + V.pw[0] * xb + V.pw[2] * zb);
+      const int    iuLo = max(int(ceilf(fminf(ua, ub) - 1e-3f)), 0);
+      const int    iuHi = min(int(floorf(fmaxf(ua, ub) + 1e-3f)), nu - 1);
+      for (int iu = iuLo; iu <= iuHi; iu++)
+      {
+        if (int(colAxis[size_t(p) * nu + iu]) != axis)
+          continue;
+        const GatherColumn C = gcols[size_t(p) * nu + iu];
+        const double       X = C.h0 + kd * C.h1;
+        const double       tt = C.t0 + kd * C.t1;
+        const double       a = V.y0 + tt * (V.sy - V.y0);
+        const double       bd = V.y1 * (1. - tt);
+        const float        bf = float(bd), rb = __builtin_amdgcn_rcpf(bf), sg = bf > 0.f ? 1.f : -1.f;
+        // rows relative to jref = the row nearest this voxel's height
+        const int          jref = int(rintf(float(double(yv) - a) * rb));
+        const float        base = float(a + bd * jref - double(yv)); // y' - j of row jref
+        const int          d0 = int(ceilf((-sg - base) * rb));
+        const float        wh = fmaxf(0.f, 1.f - fabsf(float(X) - hc));
+        const float        whc = fmaxf(0.f, 1.f - fabsf(fminf(fmaxf(float(X), -hmax), hmax) - hc));
+        for (int r = 0; r < kGRows; r++)
+        {
+          const int jv = jref + d0 + r;
+          if (jv < 0 || jv >= nv)
+            continue;
+          const size_t pix = size_t(p) * viewRays + size_t(jv) * nu + iu;
+          const uint2  ve = gend[pix];
+          const float  val = __uint_as_float(ve.x);
+          if (val == 0.f)
+            continue;
+          const unsigned en = ve.y;
+          const int      ns = int(en & 0xffffu), fs = int(en >> 16);
+          if (kidx < min(ns, fs) || kidx > max(ns, fs))
+            continue;
+          const float yabs = fmaf(bf, float(d0 + r), base) + yv;
+          float       sk = 1.f, w;
+          if (kidx == ns || kidx == fs)
+          {
+            const float2 w2 = gw[pix];
+            sk = kidx == ns ? w2.x : w2.y;
+            w = whc * fmaxf(0.f, 1.f - fabsf(fminf(fmaxf(yabs, -cy), cy) - yv));
+          }
+          else
+            w = wh * fmaxf(0.f, 1.f - fabsf(yabs - yv));
+          acc = fmaf(sk * w, val, acc);
+        }
+      }
+    }
+  }
+  vol[idx] += acc;
+}
+
 struct JosephCache
 {
   std::map<hipStream_t, DeviceBuffer<JosephView>>          views;
   std::map<hipStream_t, DeviceBuffer<float>>               volT;
   std::map<hipStream_t, DeviceBuffer<unsigned long long>> count;
+  // gather backprojection: per-view and per-column tables, block lists, per-ray data
+  std::map<hipStream_t, DeviceBuffer<GatherView>>   gviews;
+  std::map<hipStream_t, DeviceBuffer<GatherColumn>> gcols;
+  std::map<hipStream_t, DeviceBuffer<signed char>>  gaxis;
+  std::map<hipStream_t, DeviceBuffer<int>>          gblocks;
+  std::map<hipStream_t, DeviceBuffer<float>>        gval;
+  std::map<hipStream_t, DeviceBuffer<uint2>>        gend;
+  std::map<hipStream_t, DeviceBuffer<float2>>       gw;
+  std::map<hipStream_t, DeviceBuffer<int>>          gshell; // voxels of the outer layers along x and z
+  std::map<hipStream_t, std::array<int, 3>>         gshellDims;
+  std::map<hipStream_t, int>                        gshellCount;
+  // the arguments the uploaded tables were made from: a call with the same ones (every
+  // iteration of an iterative reconstruction) reuses them without host work or copies
+  std::map<hipStream_t, std::vector<double>> viewsKey;
+  std::map<hipStream_t, std::vector<double>> gplanKey;
+  std::map<hipStream_t, long long>           gplanBlocks; // -1: the gather does not apply
 };
 thread_local JosephCache g_joseph;
+
+// Host side of the gather backprojection: eligibility, per-view and per-column tables.
+struct GatherPlan
+{
+  std::vector<GatherView>   views;
+  std::vector<GatherColumn> cols;
+  std::vector<signed char>  axis;
+  std::vector<int>          blocks; // packed block coordinates (bx | bz << 10 | by << 20)
+};
+
+// True when the geometry lets the gather take the backprojection: default clips,
+// detector columns vertical in the volume (no in-plane or out-of-plane tilt), every
+// voxel between the detector and the source, and rows dense enough that kGRows rows
+// hold every ray within one voxel of a voxel's height.  Columns whose rays are not all
+// x- or all z-dominant are marked -1 (their rays take the per-ray path).
+bool
+PlanGather(const int proj_dim[3], const int vol_dim[3], const double * proj_to_vol, const double * source,
+           double inferior_clip, double superior_clip, GatherPlan & G)
+{
+  if (inferior_clip != 0. || superior_clip != 1.)
+    return false;
+  const int nu = proj_dim[0], nv = proj_dim[1], np = proj_dim[2];
+  const int nx = vol_dim[0], ny = vol_dim[1], nz = vol_dim[2];
+  if (nx < 5 || ny < 5 || nz < 5 || nx > 16 * 1023 || nz > 16 * 1023 || ny > kGNJ * 2047 || nu > 32767)
+    return false;
+  const double cx = 0.5 * (nx - 1), cy = 0.5 * (ny - 1), cz = 0.5 * (nz - 1);
+  const double ctr[3] = { cx, cy, cz };
+  G.views.assign(np, GatherView{});
+  G.cols.assign(size_t(np) * nu, GatherColumn{});
+  G.axis.assign(size_t(np) * nu, -1);
+  double minRowStep = DBL_MAX;
+  for (int p = 0; p < np; p++)
+  {
+    const double * m = proj_to_vol + 12 * p;
+    const double * S = source + 3 * p;
+    const double   eu[3] = { m[0], m[4], m[8] }, ev[3] = { m[1], m[5], m[9] };
+    const double   o[3] = { m[3] + m[2] * p, m[7] + m[6] * p, m[11] + m[10] * p };
+    const double   scale = std::max(std::abs(eu[0]) + std::abs(eu[2]), std::abs(ev[1]));
+    if (!(scale > 0.) || std::abs(eu[1]) > 1e-9 * scale || std::abs(ev[0]) > 1e-9 * scale ||
+        std::abs(ev[2]) > 1e-9 * scale)
+      return false;
+    // volume index -> detector (u w, v w, w): iu = (c x ev).d / n.d, jv = (eu x c).d / n.d
+    // with c = S - o, n = eu x ev, d = P - S (the ray from the source through P meets the
+    // detector plane o + iu eu + jv ev)
+    auto cross = [](const double * a, const double * b, double * r) {
+      r[0] = a[1] * b[2] - a[2] * b[1];
+      r[1] = a[2] * b[0] - a[0] * b[2];
+      r[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    const double c[3] = { S[0] - o[0], S[1] - o[1], S[2] - o[2] };
+    double       n[3], au[3], av[3];
+    cross(eu, ev, n);
+    cross(c, ev, au);
+    cross(eu, c, av);
+    GatherView & V = G.views[p];
+    auto         row = [&](const double * a, float * out) {
+      double off = 0.;
+      for (int r = 0; r < 3; r++)
+      {
+        out[r] = float(a[r]);
+        off += a[r] * (ctr[r] - S[r]);
+      }
+      out[3] = float(off);
+    };
+    row(au, V.pu);
+    row(av, V.pv);
+    row(n, V.pw);
+    V.y0 = o[1] - cy;
+    V.y1 = ev[1];
+    V.sy = S[1] - cy;
+    const double dyTop = std::abs(S[1] - o[1]), dyBot = std::abs(S[1] - (o[1] + (nv - 1) * ev[1]));
+    const double mh = std::max(dyTop, dyBot);
+    for (int iu = 0; iu < nu; iu++)
+    {
+      const double px = o[0] + iu * eu[0], pz = o[2] + iu * eu[2];
+      const double dx = S[0] - px, dz = S[2] - pz;
+      int          ax = -1;
+      if (std::abs(dx) >= std::abs(dz) && std::abs(dx) >= mh)
+        ax = 0;
+      else if (std::abs(dz) > std::abs(dx) && std::abs(dz) > mh)
+        ax = 2;
+      if (ax < 0)
+        continue;
+      GatherColumn & C = G.cols[size_t(p) * nu + iu];
+      const double   dk = ax == 0 ? dx : dz, dh = ax == 0 ? dz : dx;
+      const double   pk = ax == 0 ? px : pz, ph = ax == 0 ? pz : px;
+      const double   ck = ax == 0 ? cx : cz, ch = ax == 0 ? cz : cx;
+      C.t1 = 1. / dk;
+      C.t0 = (ck - pk) * C.t1;
+      C.h0 = ph - ch + C.t0 * dh;
+      C.h1 = C.t1 * dh;
+      // the column's rays cross the volume's horizontal square where t is in [tIn, tOut];
+      // there every voxel must lie between the pixel (t = 0) and the source (t = 1), and
+      // the rows' vertical spacing, |ev_y| (1 - t), is smallest at tOut
+      double tIn = -DBL_MAX, tOut = DBL_MAX;
+      const double hp[2] = { px - cx, pz - cz }, hd[2] = { dx, dz }, hc[2] = { cx, cz };
+      for (int r = 0; r < 2; r++)
+      {
+        if (hd[r] == 0.)
+        {
+          if (std::abs(hp[r]) > hc[r])
+            tIn = DBL_MAX;
+          continue;
+        }
+        double ta = (-hc[r] - hp[r]) / hd[r], tb = (hc[r] - hp[r]) / hd[r];
+        if (ta > tb)
+          std::swap(ta, tb);
+        tIn = std::max(tIn, ta);
+        tOut = std::min(tOut, tb);
+      }
+      if (tIn <= tOut)
+      {
+        if (!(tIn > 0. && tOut < 1.))
+          return false;
+        minRowStep = std::min(minRowStep, std::abs(ev[1] * (1. - tOut)));
+      }
+      G.axis[size_t(p) * nu + iu] = static_cast<signed char>(ax);
+      (ax == 0 ? V.hasX : V.hasZ) = 1;
+    }
+  }
+  // the rows within one voxel of a height form an open interval of 2 / step rows
+  if (!(minRowStep > 0.) || 2. / minRowStep > double(kGRows))
+    return false;
+  const int nbx = (nx + 15) / 16, nbz = (nz + 15) / 16, nby = (ny + kGNJ - 1) / kGNJ;
+  G.blocks.clear();
+  for (int by = 0; by < nby; by++)
+    for (int bz = 0; bz < nbz; bz++)
+      for (int bx = 0; bx < nbx; bx++)
+        G.blocks.push_back(bx | (bz << 10) | (by << 20));
+  return true;
+}
+
+// Runs the gather backprojection of every view (in chunks that keep the per-ray data,
+// 20 bytes a ray, under 1 GiB): per chunk, the per-ray pass (which also splats the rays
+// the gather does not take into acc / accT), then the gather over every block (columns
+// off the two outer layers along x and z) and the shell kernel (those layers).
+// Plans the gather for these arguments and uploads its tables, or reuses the tables of
+// the previous call on stream s when its arguments were the same.  Returns the number
+// of gather blocks, or -1 when the gather does not apply.
+long long
+PrepareGather(const int proj_dim[3], const int vol_dim[3], const double * proj_to_vol, const double * source,
+              double inferior_clip, double superior_clip, hipStream_t s)
+{
+  const int           np = proj_dim[2], nu = proj_dim[0];
+  std::vector<double> key = { double(proj_dim[0]), double(proj_dim[1]), double(np),           double(vol_dim[0]),
+                              double(vol_dim[1]),  double(vol_dim[2]),  inferior_clip,        superior_clip };
+  key.insert(key.end(), proj_to_vol, proj_to_vol + 12 * size_t(np));
+  key.insert(key.end(), source, source + 3 * size_t(np));
+  int dev = 0;
+  RTKHIP_CHECK(hipGetDevice(&dev));
+  key.push_back(double(dev));
+  auto & cached = g_joseph.gplanKey[s];
+  if (cached == key)
+    return g_joseph.gplanBlocks[s];
+  cached.clear(); // a failure below leaves no stale entry
+  GatherPlan G;
+  long long  nblocks = -1;
+  if (PlanGather(proj_dim, vol_dim, proj_to_vol, source, inferior_clip, superior_clip, G))
+  {
+    auto & gv = g_joseph.gviews[s];
+    auto & gc = g_joseph.gcols[s];
+    auto & ga = g_joseph.gaxis[s];
+    auto & gb = g_joseph.gblocks[s];
+    gv.Resize(np);
+    gc.Resize(size_t(np) * nu);
+    ga.Resize(size_t(np) * nu);
+    gb.Resize(G.blocks.size());
+    RTKHIP_CHECK(hipMemcpyAsync(gv.get(), G.views.data(), sizeof(GatherView) * np, hipMemcpyHostToDevice, s));
+    RTKHIP_CHECK(hipMemcpyAsync(gc.get(), G.cols.data(), sizeof(GatherColumn) * G.cols.size(),
+                                hipMemcpyHostToDevice, s));
+    RTKHIP_CHECK(hipMemcpyAsync(ga.get(), G.axis.data(), G.axis.size(), hipMemcpyHostToDevice, s));
+    RTKHIP_CHECK(
+      hipMemcpyAsync(gb.get(), G.blocks.data(), sizeof(int) * G.blocks.size(), hipMemcpyHostToDevice, s));
+    nblocks = (long long)G.blocks.size();
+  }
+  g_joseph.gplanBlocks[s] = nblocks;
+  cached = std::move(key);
+  return nblocks;
+}
+
+void
+GatherBackproject(long long nblocks, const JosephView * dv, const int proj_dim[3], const int vol_dim[3],
+                  const double vol_spacing[3], double infClip, double supClip, const void * dev_proj, int proj_dtype,
+                  float * vol, float * volT, hipStream_t s)
+{
+  const int nu = proj_dim[0], nv = proj_dim[1], np = proj_dim[2];
+  const int nx = vol_dim[0], ny = vol_dim[1], nz = vol_dim[2];
+  auto &    gv = g_joseph.gviews[s];
+  auto &    gc = g_joseph.gcols[s];
+  auto &    ga = g_joseph.gaxis[s];
+  auto &    gb = g_joseph.gblocks[s];
+  // the voxels of the two outer layers along x and z, listed once per volume size
+  auto &                   gs = g_joseph.gshell[s];
+  const std::array<int, 3> dims = { nx, ny, nz };
+  if (!gs.get() || g_joseph.gshellDims[s] != dims)
+  {
+    std::vector<int> shell;
+    for (int z = 0; z < nz; z++)
+      for (int y = 0; y < ny; y++)
+        for (int x = 0; x < nx; x++)
+          if (x < 2 || x > nx - 3 || z < 2 || z > nz - 3)
+            shell.push_back((z * ny + y) * nx + x);
+    gs.Resize(shell.size());
+    RTKHIP_CHECK(hipMemcpy(gs.get(), shell.data(), sizeof(int) * shell.size(), hipMemcpyHostToDevice));
+    g_joseph.gshellDims[s] = dims;
+    g_joseph.gshellCount[s] = int(shell.size());
+  }
+  const int nshell = g_joseph.gshellCount[s];
+  const size_t rays = size_t(nu) * nv;
+  const int    chunk = int(std::max<size_t>(1, std::min<size_t>(np, (size_t(1) << 30) / (20 * rays))));
+  auto &       dval = g_joseph.gval[s];
+  auto &       dend = g_joseph.gend[s];
+  auto &       dw = g_joseph.gw[s];
+  dval.Resize(rays * chunk);
+  dend.Resize(rays * chunk);
+  dw.Resize(rays * chunk);
+  for (int p0 = 0; p0 < np; p0 += chunk)
+  {
+    const int c = std::min(chunk, np - p0);
+    dim3      pg((nu + 15) / 16, c, (nv + 15) / 16);
+#define RTKHIP_GPREP(TP)                                                                                             \
+  hipLaunchKernelGGL(joseph_gather_prep_kernel<TP>, pg, dim3(256), 0, s, dv, p0, ga.get() + size_t(p0) * nu, nu, nv, \
+                     static_cast<const TP *>(dev_proj), vol, volT, nx, ny, nz, vol_spacing[0], vol_spacing[1],     \
+                     vol_spacing[2], infClip, supClip, dval.get(), dend.get(), dw.get())
+    if (proj_dtype == RTKHIP_F16)
+      RTKHIP_GPREP(__half);
+    else if (proj_dtype == RTKHIP_BF16)
+      RTKHIP_GPREP(__hip_bfloat16);
+    else
+      RTKHIP_GPREP(float);
+#undef RTKHIP_GPREP
+    RTKHIP_CHECK_LAUNCH();
+    hipLaunchKernelGGL(joseph_gather_kernel, dim3(unsigned(nblocks)), dim3(256), 0, s, gv.get() + p0,
+                       gc.get() + size_t(p0) * nu, ga.get() + size_t(p0) * nu, c, nu, nv, dval.get(), dend.get(),
+                       dw.get(), gb.get(), vol, nx, ny, nz);
+    RTKHIP_CHECK_LAUNCH();
+    hipLaunchKernelGGL(joseph_gather_shell_kernel, dim3(unsigned((nshell + 255) / 256)), dim3(256), 0, s,
+                       gv.get() + p0, gc.get() + size_t(p0) * nu, ga.get() + size_t(p0) * nu, c, nu, nv, dval.get(),
+                       dend.get(), dw.get(), gs.get(), nshell, vol, nx, ny, nz);
+    RTKHIP_CHECK_LAUNCH();
+  }
+}
 
 void
 CheckArgs(const int proj_dim[3], const int vol_dim[3], const double * proj_to_vol, const double * source,
@@ -850,6 +1585,16 @@ CheckArgs(const int proj_dim[3], const int vol_dim[3], const double * proj_to_vo
 JosephView *
 UploadViews(int np, const double * proj_to_vol, const double * source, hipStream_t s)
 {
+  auto &              dv = g_joseph.views[s];
+  auto &              cached = g_joseph.viewsKey[s];
+  std::vector<double> key(proj_to_vol, proj_to_vol + 12 * size_t(np));
+  key.insert(key.end(), source, source + 3 * size_t(np));
+  int dev = 0;
+  RTKHIP_CHECK(hipGetDevice(&dev));
+  key.push_back(double(dev));
+  if (dv.get() && cached == key)
+    return dv.get();
+  cached.clear();
   std::vector<JosephView> hv(np);
   for (int p = 0; p < np; p++)
   {
@@ -859,9 +1604,9 @@ UploadViews(int np, const double * proj_to_vol, const double * source, hipStream
       hv[p].src[j] = source[3 * p + j];
     hv[p].src[3] = 1.;
   }
-  auto & dv = g_joseph.views[s];
   dv.Resize(np);
   RTKHIP_CHECK(hipMemcpyAsync(dv.get(), hv.data(), sizeof(JosephView) * np, hipMemcpyHostToDevice, s));
+  cached = std::move(key);
   return dv.get();
 }
 
@@ -950,7 +1695,18 @@ rtkhip_joseph_backproject(const int proj_dim[3], const int vol_dim[3], const dou
     dT.Resize(nvox);
     RTKHIP_CHECK(hipMemsetAsync(dT.get(), 0, nvox * sizeof(float), s));
     const double infClip = 1. - superior_clip, supClip = 1. - inferior_clip;
-    dim3         grid((nu + 15) / 16, np, (nv + 15) / 16);
+    const long long nblocks = PrepareGather(proj_dim, vol_dim, proj_to_vol, source, inferior_clip, superior_clip, s);
+    if (nblocks >= 0)
+    {
+      GatherBackproject(nblocks, dv, proj_dim, vol_dim, vol_spacing, infClip, supClip, dev_proj, proj_dtype,
+                        dev_vol_out, dT.get(), s);
+      dim3 tg((vol_dim[0] + 31) / 32, (vol_dim[2] + 31) / 32, vol_dim[1]);
+      hipLaunchKernelGGL(add_transposed_kernel, tg, dim3(256), 0, s, dT.get(), dev_vol_out, vol_dim[0], vol_dim[1],
+                         vol_dim[2]);
+      RTKHIP_CHECK_LAUNCH();
+      return;
+    }
+    dim3 grid((nu + 15) / 16, np, (nv + 15) / 16);
 #define RTKHIP_JBP_LAUNCH(TP)                                                                                      \
   hipLaunchKernelGGL(joseph_bp_tile_kernel<TP>, grid, dim3(256), 0, s, dv, nu, nv, static_cast<const TP *>(dev_proj), \
                      dev_vol_out, dT.get(), vol_dim[0], vol_dim[1], vol_dim[2], vol_spacing[0], vol_spacing[1],       \

@@ -141,6 +141,21 @@ class WarmStartCNN:
                                         ctypes.byref(loss), _stream()))
         return loss.value, y
 
+    def backward_to(self, target):
+        """Backward pass of the last forward toward `target` (the gradient of
+        mean((scale (y - target))^2)); returns that loss."""
+        t = self._batched(target)
+        loss = ctypes.c_double()
+        check(lib().rtkhip_cnn_backward(self._h, ctypes.c_void_p(self._y.data_ptr()), ctypes.c_void_p(t.data_ptr()),
+                                        ctypes.byref(loss), _stream()))
+        return loss.value
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        """One Adam update from the last backward pass."""
+        self.step += 1
+        check(lib().rtkhip_cnn_adam_step(self._h, ctypes.c_float(lr), ctypes.c_float(beta1), ctypes.c_float(beta2),
+                                         ctypes.c_float(eps), self.step, _stream()))
+
     def train_step(self, x, target, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
         """One Adam step on mean((scale (net(x) - target))^2); returns the loss."""
         loss, _ = self.loss_and_grads(x, target)

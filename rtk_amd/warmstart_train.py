@@ -112,6 +112,82 @@ def train(net, pairs, steps, crop=64, batch=2, lr=1e-3, seed=0):
     return losses
 
 
+def apply_normal_operator(g, x, stack_like):
+    """A x = BP_Joseph(FP_Joseph(x)) with unit weights: the operator of the CG runs
+    (ReconstructionConjugateGradientOperator with the matched Joseph pair)."""
+    fp = rtk.HipForwardProjectionImageFilter()
+    fp.SetInput(0, stack_like.like(torch.zeros_like(stack_like.tensor)))
+    fp.SetInput(1, x)
+    fp.SetGeometry(g)
+    fp.Update()
+    bp = rtk.HipJosephBackProjectionImageFilter()
+    bp.SetInput(0, x.like(torch.zeros_like(x.tensor)))
+    bp.SetInput(1, fp.GetOutput())
+    bp.SetGeometry(g)
+    bp.Update()
+    return bp.GetOutput().tensor
+
+
+def make_full_sample(g, seed, cg_iter=40):
+    """(FDK, B = BP_Joseph(p), cg_iter-iteration CG volume, the stack image) of one seed."""
+    vol = volume_source()
+    proj = noisy_projections(g, seed)
+    x_fdk = fdk(g, proj, vol)
+    bp = rtk.HipJosephBackProjectionImageFilter()
+    bp.SetInput(0, vol)
+    bp.SetInput(1, proj)
+    bp.SetGeometry(g)
+    bp.Update()
+    x_cg, res, _ = cg(g, proj, vol, cg_iter)
+    return x_fdk.tensor, bp.GetOutput().tensor.clone(), x_cg.tensor, proj
+
+
+def finetune_residual(net, g, samples, steps, lam_img=1.0, lam_res=1.0, lr=2e-4, seed=0, crop=128, halo=6):
+    """Whole-volume objective loss = lam_img mean((s (y - t))^2) + lam_res ||B - A y||^2 / ||B||^2
+    with y = net(FDK), t the CG target and A the CG operator (the matched Joseph pair):
+    the second term is the start's squared relative residual, the quantity CG's stopping
+    test reads.  Its gradient, -2 lam_res A (B - A y) / ||B||^2 (A is symmetric), costs two
+    applications of A to the whole-volume output per step.  The network's parameters
+    then follow that output gradient on a random crop^3 block (a stochastic estimate of
+    the whole-volume parameter gradient, scaled to it): the block is run with `halo`
+    voxels of context (the network's receptive radius is 6, so its centre equals the
+    whole-volume output) and the gradient enters the network's own backward pass,
+    which differentiates the first term's form, through the target t' that gives the
+    same output gradient, t' = y - (n / (2 s^2)) dL/dy (t' = y on the halo)."""
+    rng = np.random.default_rng(seed)
+    stats = []
+    vol_like = volume_source()
+    s2 = net.scale * net.scale
+    for it in range(steps):
+        x, b, t, stack = samples[rng.integers(len(samples))]
+        y = net.forward(x)
+        r = b - apply_normal_operator(g, vol_like.like(y), stack)
+        ar = apply_normal_operator(g, vol_like.like(r.contiguous()), stack)
+        bb = float(torch.dot(b.ravel().double(), b.ravel().double()))
+        rel2 = float(torch.dot(r.ravel().double(), r.ravel().double())) / bb
+        n_full = y.numel()
+        grad = (lam_img * 2. * s2 / n_full) * (y - t) - (2. * lam_res / bb) * ar  # dL/dy
+        img = float(torch.mean((net.scale * (y - t)) ** 2))
+        Z, Y, X = y.shape
+        c = [min(crop, n) for n in (Z, Y, X)]
+        o = [int(rng.integers(0, n - cn + 1)) for n, cn in zip((Z, Y, X), c)]
+        lo = [max(q - halo, 0) for q in o]
+        hi = [min(q + cn + halo, n) for q, cn, n in zip(o, c, (Z, Y, X))]
+        xc = x[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]].contiguous()
+        yc = net.forward(xc)
+        inner = tuple(slice(q - l, q - l + cn) for q, l, cn in zip(o, lo, c))
+        gc = torch.zeros_like(yc)
+        gc[inner] = grad[o[0]:o[0] + c[0], o[1]:o[1] + c[1], o[2]:o[2] + c[2]] * (n_full / float(np.prod(c)))
+        net.backward_to((yc - (yc.numel() / (2. * s2)) * gc).contiguous())
+        step_lr = lr * 0.5 * (1 + np.cos(np.pi * it / steps))
+        net.adam(step_lr)
+        stats.append((img, rel2))
+        if (it + 1) % 20 == 0:
+            m = np.mean(stats[-20:], axis=0)
+            log(f"finetune step {it + 1}/{steps}: image loss {m[0]:.4g}, start rel residual {np.sqrt(m[1]):.4g}")
+    return stats
+
+
 def evaluate_seed(net, g, seed, tol=1e-3, max_iter=200):
     """CG time to relative residual `tol` from zero, FDK and CNN(FDK) starts on one
     phantom seed; the start's own cost (FDK, CNN inference) is part of its time."""
@@ -239,6 +315,12 @@ def main():
     ap.add_argument("--scale-train", type=int, default=0,
                     help="train a network on this many 512^3 / 180-view pairs (toward the --cg-iter volume)")
     ap.add_argument("--weights", default=None, help="network for --eval-only / --scale-eval (default: shipped)")
+    ap.add_argument("--finetune", type=int, default=0,
+                    help="whole-volume fine-tuning steps of the --weights network with the residual term")
+    ap.add_argument("--ft-samples", type=int, default=48, help="training seeds for --finetune")
+    ap.add_argument("--lam-img", type=float, default=1.0)
+    ap.add_argument("--lam-res", type=float, default=1.0)
+    ap.add_argument("--ft-crop", type=int, default=128, help="block size of --finetune's parameter gradient")
     args = ap.parse_args()
     rtk.lib()
     if args.scale_train or args.scale_eval:
@@ -268,6 +350,33 @@ def main():
                           "eval": rows}), flush=True)
         return
     g = geometry()
+    if args.finetune:
+        from rtk_amd.warmstart import default_weights_path
+        net = WarmStartCNN.load(args.weights or default_weights_path())
+        t0 = time.perf_counter()
+        samples = [make_full_sample(g, TRAIN_SEED0 + i, args.cg_iter) for i in range(args.ft_samples)]
+        t1 = time.perf_counter()
+        log(f"{args.ft_samples} whole-volume samples in {t1 - t0:.0f} s")
+        stats = finetune_residual(net, g, samples, args.finetune, args.lam_img, args.lam_res, args.lr,
+                                  crop=args.ft_crop)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        del samples
+        net.save(args.out)
+        rows = evaluate(net, g, [EVAL_SEED0 + i for i in range(args.eval)])
+        rep = {"finetune_steps": args.finetune, "samples": args.ft_samples, "lam_img": args.lam_img,
+               "lam_res": args.lam_res, "lr": args.lr, "cg_iter": args.cg_iter, "data_seconds": t1 - t0,
+               "train_seconds": t2 - t1, "first": list(np.mean(stats[:20], axis=0)),
+               "last": list(np.mean(stats[-20:], axis=0)), "eval": rows}
+        for name in ("zero", "fdk", "cnn"):
+            rep[f"median_{name}_seconds"] = float(np.median([r[name]["seconds"] for r in rows]))
+            rep[f"median_{name}_iterations"] = float(np.median([r[name]["iterations"] for r in rows]))
+            rep[f"median_{name}_rmse_vs_phantom"] = float(np.median([r[name]["rmse_vs_phantom"] for r in rows]))
+        rep["speedup_cnn_vs_zero"] = rep["median_zero_seconds"] / rep["median_cnn_seconds"]
+        with open(args.report, "w") as f:
+            json.dump(rep, f, indent=1)
+        log(json.dumps({k: v for k, v in rep.items() if k != "eval"}))
+        return
     if args.eval_only:
         from rtk_amd.warmstart import default_weights_path
         net = WarmStartCNN.load(args.weights or default_weights_path())
