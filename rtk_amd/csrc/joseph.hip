@@ -962,7 +962,8 @@ StepWeight(int k, unsigned e, float2 w)
 // receive middle steps only; the two outer layers along y (a choice uniform across the
 // workgroup) also read each candidate ray's first/last slices and weights from global
 // memory.  Columns of the two outer layers along x or z are joseph_gather_shell_kernel's.
-__global__ __launch_bounds__(256) void joseph_gather_kernel(
+template <int G>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void joseph_gather_kernel(
   const GatherView * __restrict__ gviews, const GatherColumn * __restrict__ gcols,
   const signed char * __restrict__ colAxis, int nviews, int nu, int nv, const float * __restrict__ gval,
   const uint2 * __restrict__ gend, const float2 * __restrict__ gw, const int * __restrict__ blocks, float * vol,
@@ -1108,37 +1109,55 @@ __global__ __launch_bounds__(256) void joseph_gather_kernel(
             const float wh = fmaxf(0.f, 1.f - fabsf(float(X - double(hc))));
             const int   rowBase = jref - cur.y; // LDS row of jref
             const int   kidx = axis == 0 ? i : k;   // this voxel column's slice index
+            // R rows hold every ray within one voxel of a height: 4 where the row step is
+            // at least half a voxel (on every lane of the wave), else 5.  The first G rows
+            // lie inside the window by construction (|y' - j| < 1), so their weight needs
+            // no clamp at zero.
+            // first row with |y' - j| < 1 for voxel jj: ceil((-sg - (e - jj)) / b)
+            const float q0 = (-sg - e) * rb;
+            // the staged rows of the tile (a block's candidate rows lie in it: the tile
+            // holds the widened block's footprint with kGRows + 1 rows of margin)
+            const float * tcol = &tile[b][rowBase * kGStride + lc];
+            auto middle = [&](auto rowsTag) {
+              constexpr int R = decltype(rowsTag)::value;
 #pragma unroll
-            for (int jj = 0; jj < kGNJ; jj++)
-            {
-              const float base = e - float(jj);
-              const int   d0 = int(ceilf((-sg - base) * rb)); // first row with |y' - j| < 1
-              const float y0 = fmaf(bf, float(d0), base);     // y' - j of that row
-              float       s = 0.f;
-              // voxels of the outer layers along y: below (uniform)
-              if (j0 + jj < 2 || j0 + jj > ny - 3)
-                continue;
-              if constexpr (STAGED)
+              for (int jj = 0; jj < kGNJ; jj++)
               {
-                const int     r0 = min(max(rowBase + d0, 0), kGTileV - kGRows);
-                const float * src = &tile[b][r0 * kGStride + lc];
-#pragma unroll
-                for (int r = 0; r < kGRows; r++)
-                  s = fmaf(fmaxf(0.f, 1.f - fabsf(fmaf(bf, float(r), y0))), src[r * kGStride], s);
-              }
-              else
-              {
-#pragma unroll
-                for (int r = 0; r < kGRows; r++)
+                // voxels of the outer layers along y: below (uniform)
+                if (j0 + jj < 2 || j0 + jj > ny - 3)
+                  continue;
+                const float d0f = ceilf(fmaf(float(jj), rb, q0));
+                const int   d0 = int(d0f);
+                const float y0 = fmaf(bf, d0f, e - float(jj)); // y' - j of that row
+                float       s = 0.f;
+                auto        weight = [&](int r) {
+                  const float w = 1.f - fabsf(r == 0 ? y0 : fmaf(bf, float(r), y0));
+                  return r < G ? w : fmaxf(0.f, w);
+                };
+                if constexpr (STAGED)
                 {
-                  const int jv = jref + d0 + r;
-                  if (jv >= 0 && jv < nv)
-                    s = fmaf(fmaxf(0.f, 1.f - fabsf(fmaf(bf, float(r), y0))),
-                             gval[size_t(p) * viewRays + size_t(jv) * nu + iu], s);
+                  const float * src = tcol + __mul24(d0, kGStride);
+#pragma unroll
+                  for (int r = 0; r < R; r++)
+                    s = fmaf(weight(r), src[r * kGStride], s);
                 }
+                else
+                {
+#pragma unroll
+                  for (int r = 0; r < R; r++)
+                  {
+                    const int jv = jref + d0 + r;
+                    if (jv >= 0 && jv < nv)
+                      s = fmaf(weight(r), gval[size_t(p) * viewRays + size_t(jv) * nu + iu], s);
+                  }
+                }
+                acc[jj] = fmaf(wh, s, acc[jj]);
               }
-              acc[jj] = fmaf(wh, s, acc[jj]);
-            }
+            };
+            if (__all(fabsf(bf) >= 0.5f))
+              middle(std::integral_constant<int, 4>{});
+            else
+              middle(std::integral_constant<int, kGRows>{});
             if (yshell)
             {
               // rows 0, 1, ny - 2, ny - 1 (those in this block, each once): the step weight
@@ -1253,9 +1272,11 @@ __global__ __launch_bounds__(256) void joseph_gather_shell_kernel(
       const int    iuHi = min(int(floorf(fmaxf(ua, ub) + 1e-3f)), nu - 1);
       for (int iu = iuLo; iu <= iuHi; iu++)
       {
-        if (int(colAxis[size_t(p) * nu + iu]) != axis)
-          continue;
+        // both loads issued before the test, so they overlap
+        const int          cax = int(colAxis[size_t(p) * nu + iu]);
         const GatherColumn C = gcols[size_t(p) * nu + iu];
+        if (cax != axis)
+          continue;
         const double       X = C.h0 + kd * C.h1;
         const double       tt = C.t0 + kd * C.t1;
         const double       a = V.y0 + tt * (V.sy - V.y0);
@@ -1272,6 +1293,12 @@ __global__ __launch_bounds__(256) void joseph_gather_shell_kernel(
           const int jv = jref + d0 + r;
           if (jv < 0 || jv >= nv)
             continue;
+          // the middle-step and end-step weights of this row; no read when both are zero
+          const float yabs = fmaf(bf, float(d0 + r), base) + yv;
+          const float wm = wh * fmaxf(0.f, 1.f - fabsf(yabs - yv));
+          const float we = whc * fmaxf(0.f, 1.f - fabsf(fminf(fmaxf(yabs, -cy), cy) - yv));
+          if (wm == 0.f && we == 0.f)
+            continue;
           const size_t pix = size_t(p) * viewRays + size_t(jv) * nu + iu;
           const uint2  ve = gend[pix];
           const float  val = __uint_as_float(ve.x);
@@ -1281,16 +1308,13 @@ __global__ __launch_bounds__(256) void joseph_gather_shell_kernel(
           const int      ns = int(en & 0xffffu), fs = int(en >> 16);
           if (kidx < min(ns, fs) || kidx > max(ns, fs))
             continue;
-          const float yabs = fmaf(bf, float(d0 + r), base) + yv;
-          float       sk = 1.f, w;
+          float sk = 1.f, w = wm;
           if (kidx == ns || kidx == fs)
           {
             const float2 w2 = gw[pix];
             sk = kidx == ns ? w2.x : w2.y;
-            w = whc * fmaxf(0.f, 1.f - fabsf(fminf(fmaxf(yabs, -cy), cy) - yv));
+            w = we;
           }
-          else
-            w = wh * fmaxf(0.f, 1.f - fabsf(yabs - yv));
           acc = fmaf(sk * w, val, acc);
         }
       }
@@ -1320,6 +1344,7 @@ struct JosephCache
   std::map<hipStream_t, std::vector<double>> viewsKey;
   std::map<hipStream_t, std::vector<double>> gplanKey;
   std::map<hipStream_t, long long>           gplanBlocks; // -1: the gather does not apply
+  std::map<hipStream_t, int>                 gplanFree;
 };
 thread_local JosephCache g_joseph;
 
@@ -1330,6 +1355,7 @@ struct GatherPlan
   std::vector<GatherColumn> cols;
   std::vector<signed char>  axis;
   std::vector<int>          blocks; // packed block coordinates (bx | bz << 10 | by << 20)
+  int                       freeRows = 1; // leading candidate rows whose weight needs no clamp
 };
 
 // True when the geometry lets the gather take the backprojection: default clips,
@@ -1352,7 +1378,7 @@ PlanGather(const int proj_dim[3], const int vol_dim[3], const double * proj_to_v
   G.views.assign(np, GatherView{});
   G.cols.assign(size_t(np) * nu, GatherColumn{});
   G.axis.assign(size_t(np) * nu, -1);
-  double minRowStep = DBL_MAX;
+  double minRowStep = DBL_MAX, maxRowStep = 0.;
   for (int p = 0; p < np; p++)
   {
     const double * m = proj_to_vol + 12 * p;
@@ -1413,11 +1439,12 @@ PlanGather(const int proj_dim[3], const int vol_dim[3], const double * proj_to_v
       C.t0 = (ck - pk) * C.t1;
       C.h0 = ph - ch + C.t0 * dh;
       C.h1 = C.t1 * dh;
-      // the column's rays cross the volume's horizontal square where t is in [tIn, tOut];
+      // the column's rays cross the volume's horizontal square, widened by the one voxel
+      // by which a candidate's position may lie outside it, where t is in [tIn, tOut];
       // there every voxel must lie between the pixel (t = 0) and the source (t = 1), and
-      // the rows' vertical spacing, |ev_y| (1 - t), is smallest at tOut
+      // the rows' vertical spacing, |ev_y| (1 - t), is smallest at tOut, largest at tIn
       double tIn = -DBL_MAX, tOut = DBL_MAX;
-      const double hp[2] = { px - cx, pz - cz }, hd[2] = { dx, dz }, hc[2] = { cx, cz };
+      const double hp[2] = { px - cx, pz - cz }, hd[2] = { dx, dz }, hc[2] = { cx + 1., cz + 1. };
       for (int r = 0; r < 2; r++)
       {
         if (hd[r] == 0.)
@@ -1437,6 +1464,7 @@ PlanGather(const int proj_dim[3], const int vol_dim[3], const double * proj_to_v
         if (!(tIn > 0. && tOut < 1.))
           return false;
         minRowStep = std::min(minRowStep, std::abs(ev[1] * (1. - tOut)));
+        maxRowStep = std::max(maxRowStep, std::abs(ev[1] * (1. - tIn)));
       }
       G.axis[size_t(p) * nu + iu] = static_cast<signed char>(ax);
       (ax == 0 ? V.hasX : V.hasZ) = 1;
@@ -1445,6 +1473,9 @@ PlanGather(const int proj_dim[3], const int vol_dim[3], const double * proj_to_v
   // the rows within one voxel of a height form an open interval of 2 / step rows
   if (!(minRowStep > 0.) || 2. / minRowStep > double(kGRows))
     return false;
+  // the first candidate row has |y' - j| < 1 and rows advance by the step b, so rows
+  // 0..2 stay inside the window when 3 |b| < 2 (a margin covers the FP32 arithmetic)
+  G.freeRows = maxRowStep < (2. / 3.) * (1. - 1e-4) ? 3 : 1;
   const int nbx = (nx + 15) / 16, nbz = (nz + 15) / 16, nby = (ny + kGNJ - 1) / kGNJ;
   G.blocks.clear();
   for (int by = 0; by < nby; by++)
@@ -1463,7 +1494,7 @@ PlanGather(const int proj_dim[3], const int vol_dim[3], const double * proj_to_v
 // of gather blocks, or -1 when the gather does not apply.
 long long
 PrepareGather(const int proj_dim[3], const int vol_dim[3], const double * proj_to_vol, const double * source,
-              double inferior_clip, double superior_clip, hipStream_t s)
+              double inferior_clip, double superior_clip, hipStream_t s, int & freeRows)
 {
   const int           np = proj_dim[2], nu = proj_dim[0];
   std::vector<double> key = { double(proj_dim[0]), double(proj_dim[1]), double(np),           double(vol_dim[0]),
@@ -1475,7 +1506,10 @@ PrepareGather(const int proj_dim[3], const int vol_dim[3], const double * proj_t
   key.push_back(double(dev));
   auto & cached = g_joseph.gplanKey[s];
   if (cached == key)
+  {
+    freeRows = g_joseph.gplanFree[s];
     return g_joseph.gplanBlocks[s];
+  }
   cached.clear(); // a failure below leaves no stale entry
   GatherPlan G;
   long long  nblocks = -1;
@@ -1498,12 +1532,13 @@ PrepareGather(const int proj_dim[3], const int vol_dim[3], const double * proj_t
     nblocks = (long long)G.blocks.size();
   }
   g_joseph.gplanBlocks[s] = nblocks;
+  g_joseph.gplanFree[s] = freeRows = G.freeRows;
   cached = std::move(key);
   return nblocks;
 }
 
 void
-GatherBackproject(long long nblocks, const JosephView * dv, const int proj_dim[3], const int vol_dim[3],
+GatherBackproject(long long nblocks, int freeRows, const JosephView * dv, const int proj_dim[3], const int vol_dim[3],
                   const double vol_spacing[3], double infClip, double supClip, const void * dev_proj, int proj_dtype,
                   float * vol, float * volT, hipStream_t s)
 {
@@ -1518,12 +1553,18 @@ GatherBackproject(long long nblocks, const JosephView * dv, const int proj_dim[3
   const std::array<int, 3> dims = { nx, ny, nz };
   if (!gs.get() || g_joseph.gshellDims[s] != dims)
   {
+    // consecutive threads take neighbouring voxels of one row across the face (x on the
+    // faces z = 0, 1, nz - 2, nz - 1; z on the faces x = 0, 1, nx - 2, nx - 1), whose
+    // candidate rays lie in the same detector rows
     std::vector<int> shell;
-    for (int z = 0; z < nz; z++)
+    for (int z : { 0, 1, nz - 2, nz - 1 })
       for (int y = 0; y < ny; y++)
         for (int x = 0; x < nx; x++)
-          if (x < 2 || x > nx - 3 || z < 2 || z > nz - 3)
-            shell.push_back((z * ny + y) * nx + x);
+          shell.push_back((z * ny + y) * nx + x);
+    for (int x : { 0, 1, nx - 2, nx - 1 })
+      for (int y = 0; y < ny; y++)
+        for (int z = 2; z < nz - 2; z++)
+          shell.push_back((z * ny + y) * nx + x);
     gs.Resize(shell.size());
     RTKHIP_CHECK(hipMemcpy(gs.get(), shell.data(), sizeof(int) * shell.size(), hipMemcpyHostToDevice));
     g_joseph.gshellDims[s] = dims;
@@ -1554,9 +1595,9 @@ GatherBackproject(long long nblocks, const JosephView * dv, const int proj_dim[3
       RTKHIP_GPREP(float);
 #undef RTKHIP_GPREP
     RTKHIP_CHECK_LAUNCH();
-    hipLaunchKernelGGL(joseph_gather_kernel, dim3(unsigned(nblocks)), dim3(256), 0, s, gv.get() + p0,
-                       gc.get() + size_t(p0) * nu, ga.get() + size_t(p0) * nu, c, nu, nv, dval.get(), dend.get(),
-                       dw.get(), gb.get(), vol, nx, ny, nz);
+    hipLaunchKernelGGL(freeRows == 3 ? joseph_gather_kernel<3> : joseph_gather_kernel<1>, dim3(unsigned(nblocks)),
+                       dim3(256), 0, s, gv.get() + p0, gc.get() + size_t(p0) * nu, ga.get() + size_t(p0) * nu, c, nu,
+                       nv, dval.get(), dend.get(), dw.get(), gb.get(), vol, nx, ny, nz);
     RTKHIP_CHECK_LAUNCH();
     hipLaunchKernelGGL(joseph_gather_shell_kernel, dim3(unsigned((nshell + 255) / 256)), dim3(256), 0, s,
                        gv.get() + p0, gc.get() + size_t(p0) * nu, ga.get() + size_t(p0) * nu, c, nu, nv, dval.get(),
@@ -1695,10 +1736,12 @@ rtkhip_joseph_backproject(const int proj_dim[3], const int vol_dim[3], const dou
     dT.Resize(nvox);
     RTKHIP_CHECK(hipMemsetAsync(dT.get(), 0, nvox * sizeof(float), s));
     const double infClip = 1. - superior_clip, supClip = 1. - inferior_clip;
-    const long long nblocks = PrepareGather(proj_dim, vol_dim, proj_to_vol, source, inferior_clip, superior_clip, s);
+    int             freeRows = 1;
+    const long long nblocks =
+      PrepareGather(proj_dim, vol_dim, proj_to_vol, source, inferior_clip, superior_clip, s, freeRows);
     if (nblocks >= 0)
     {
-      GatherBackproject(nblocks, dv, proj_dim, vol_dim, vol_spacing, infClip, supClip, dev_proj, proj_dtype,
+      GatherBackproject(nblocks, freeRows, dv, proj_dim, vol_dim, vol_spacing, infClip, supClip, dev_proj, proj_dtype,
                         dev_vol_out, dT.get(), s);
       dim3 tg((vol_dim[0] + 31) / 32, (vol_dim[2] + 31) / 32, vol_dim[1]);
       hipLaunchKernelGGL(add_transposed_kernel, tg, dim3(256), 0, s, dT.get(), dev_vol_out, vol_dim[0], vol_dim[1],
