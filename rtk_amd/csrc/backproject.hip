@@ -46,7 +46,11 @@ struct alignas(16) ViewCoef
 
 // Column kernel, for views whose u and w do not depend on y (c[2] == c[10] == 0)
 // and that use the OPTIMIZED border rule (RTK's OptimizedBackprojectionY case).
-constexpr int kTileU = 64, kTileV = 48, kTileStride = 65; // staged detector footprint, pixels
+// The row stride, 76 floats (12 banks past a multiple of 64), spreads the taps of a
+// wave's 8 x 8 voxel patch, whose detector columns span about 16 and whose rows differ
+// by one or two, over distinct LDS banks; it is a multiple of 4, so a lane stages its
+// four pixels with one 16-byte store.
+constexpr int kTileU = 64, kTileV = 48, kTileStride = 76; // staged detector footprint, pixels
 constexpr int kStagePasses = (kTileV + 15) / 16;           // 16 rows per staging pass
 constexpr int kPlaceViews = 64;                            // footprints placed per chunk, one per lane
 // 48 rows hold a 16-voxel column's taps up to 2.6 detector rows per voxel (the
@@ -136,7 +140,7 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
                                                      const float * vol_in, float * vol_out, int nx, int ny, int nz,
                                                      float ci, float cj, float ck)
 {
-  __shared__ float stage[2][kTileV * kTileStride];
+  __shared__ __align__(16) float stage[2][kTileV * kTileStride];
   __shared__ int4  rects[2][kPlaceViews]; // footprints (u0, v0, W, H) of two chunks of views
   const int        t = threadIdx.x;
   const int        bx0 = blockIdx.x * 16, bz0 = blockIdx.y * 16, j0 = blockIdx.z * NJ;
