@@ -1012,9 +1012,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void j
   auto store = [&](int b, int p, int4 r) {
 #pragma unroll
     for (int q = 0; q < 4; q++)
+      if (sr + 16 * q < kGTileV)
 #pragma unroll
-      for (int e = 0; e < 4; e++)
-        tile[b][(sr + 16 * q) * kGStride + sc + e] = sreg[4 * q + e];
+        for (int e = 0; e < 4; e++)
+          tile[b][(sr + 16 * q) * kGStride + sc + e] = sreg[4 * q + e];
     if (t < kGTileU && r.z > 0 && t < r.z)
     {
       cols[b][t] = gcols[size_t(p) * nu + r.x + t];

@@ -19,6 +19,7 @@
 #include <hip/hip_fp16.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -92,6 +93,25 @@ __device__ __forceinline__ void
 StoreOut(__hip_bfloat16 * p, long i, float v)
 {
   p[i] = __float2bfloat16(v);
+}
+// Two adjacent outputs (i even) in one store.
+__device__ __forceinline__ void
+StoreOut2(float * p, long i, float a, float b)
+{
+  *reinterpret_cast<float2 *>(p + i) = make_float2(a, b);
+}
+__device__ __forceinline__ void
+StoreOut2(__half * p, long i, float a, float b)
+{
+  *reinterpret_cast<__half2 *>(p + i) = __floats2half2_rn(a, b);
+}
+__device__ __forceinline__ void
+StoreOut2(__hip_bfloat16 * p, long i, float a, float b)
+{
+  __hip_bfloat16 v[2] = { __float2bfloat16(a), __float2bfloat16(b) };
+  unsigned       u;
+  __builtin_memcpy(&u, v, 4);
+  *reinterpret_cast<unsigned *>(p + i) = u;
 }
 
 // In-place DIT FFT of M points in LDS whose input is stored in bit-reversed order;
@@ -512,6 +532,315 @@ __global__ __launch_bounds__(64) void ramp_wave_kernel(const float * __restrict_
       StoreOut(orow, i0 + 1, z.y * scale);
   };
   StockhamFFT<M, true>(lload, gstore, lload, lstore, tw, lane);
+}
+
+// ------------------------------------------------- persistent wavefront FFT --
+// The twiddles of one Stockham pass for the butterflies of one lane, held in
+// registers: w[b][r - 1] = W_M^{r k M / (NS R)} for butterfly j = lane + 64 b and
+// k = j mod NS (the factors StockhamPass reads from its table).
+// With several butterflies per lane (B > 1, the last pass: NS R = M, k = lane + 64 b)
+// only W^{r lane} is kept per lane and W^{r k} = W^{r lane} W^{64 r b}, whose second
+// factor is the same on every lane (scalar registers).
+template <int M, int R, int NS>
+struct PassTwiddles
+{
+  static constexpr int B = M / (R * 64);
+  static_assert(B == 1 || NS * R == M, "several butterflies per lane only in the last pass");
+  float2 base[R - 1];
+  float2 fac[B][R - 1];
+  __device__ __forceinline__ void load(const float2 * __restrict__ tw, int lane)
+  {
+    const int k = lane & (NS - 1);
+#pragma unroll
+    for (int r = 1; r < R; r++)
+    {
+      base[r - 1] = tw[r * k * (M / (NS * R))];
+#pragma unroll
+      for (int b = 1; b < B; b++)
+        fac[b][r - 1] = tw[(r * 64 * b) & (M - 1)];
+    }
+  }
+  __device__ __forceinline__ float2 w(int b, int r) const
+  {
+    return b == 0 ? base[r - 1] : cmul(base[r - 1], fac[b][r - 1]);
+  }
+};
+
+// StockhamPass with the twiddles from registers (NS > 1).
+template <int M, int R, int NS, bool INV, typename LoadF, typename StoreF>
+__device__ __forceinline__ void
+StockhamPassReg(LoadF && load, StoreF && store, const PassTwiddles<M, R, NS> & T, int lane)
+{
+  constexpr int B = M / (R * 64);
+  float2        v[B][R];
+#pragma unroll
+  for (int b = 0; b < B; b++)
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      v[b][r] = load(lane + 64 * b + r * (M / R));
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < B; b++)
+  {
+    const int j = lane + 64 * b, k = j & (NS - 1);
+#pragma unroll
+    for (int r = 1; r < R; r++)
+    {
+      float2 w = T.w(b, r);
+      if (INV)
+        w.y = -w.y;
+      v[b][r] = cmul(v[b][r], w);
+    }
+    DftReg<R, INV>(v[b]);
+    const int d = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      store(d + r * NS, v[b][r]);
+  }
+}
+
+// The passes of StockhamFFT for M = 256, 512, 1024 with every twiddle of the lane in
+// registers (9, 14 and 27 complex values).
+template <int M>
+struct WaveTwiddles;
+template <>
+struct WaveTwiddles<256>
+{
+  PassTwiddles<256, 4, 4>  a;
+  PassTwiddles<256, 4, 16> b;
+  PassTwiddles<256, 4, 64> c;
+  __device__ __forceinline__ void load(const float2 * tw, int lane)
+  {
+    a.load(tw, lane);
+    b.load(tw, lane);
+    c.load(tw, lane);
+  }
+  template <bool INV, typename F, typename L, typename LL, typename LS>
+  __device__ __forceinline__ void fft(F && first, L && last, LL && lload, LS && lstore, const float2 * tw,
+                                      int lane) const
+  {
+    StockhamPass<256, 4, 1, INV>(first, lstore, tw, lane);
+    __syncthreads();
+    StockhamPassReg<256, 4, 4, INV>(lload, lstore, a, lane);
+    __syncthreads();
+    StockhamPassReg<256, 4, 16, INV>(lload, lstore, b, lane);
+    __syncthreads();
+    StockhamPassReg<256, 4, 64, INV>(lload, last, c, lane);
+  }
+};
+template <>
+struct WaveTwiddles<512>
+{
+  PassTwiddles<512, 8, 8>  a;
+  PassTwiddles<512, 8, 64> b;
+  __device__ __forceinline__ void load(const float2 * tw, int lane)
+  {
+    a.load(tw, lane);
+    b.load(tw, lane);
+  }
+  template <bool INV, typename F, typename L, typename LL, typename LS>
+  __device__ __forceinline__ void fft(F && first, L && last, LL && lload, LS && lstore, const float2 * tw,
+                                      int lane) const
+  {
+    StockhamPass<512, 8, 1, INV>(first, lstore, tw, lane);
+    __syncthreads();
+    StockhamPassReg<512, 8, 8, INV>(lload, lstore, a, lane);
+    __syncthreads();
+    StockhamPassReg<512, 8, 64, INV>(lload, last, b, lane);
+  }
+};
+template <>
+struct WaveTwiddles<1024>
+{
+  PassTwiddles<1024, 16, 16> a;
+  PassTwiddles<1024, 4, 256> b;
+  __device__ __forceinline__ void load(const float2 * tw, int lane)
+  {
+    a.load(tw, lane);
+    b.load(tw, lane);
+  }
+  template <bool INV, typename F, typename L, typename LL, typename LS>
+  __device__ __forceinline__ void fft(F && first, L && last, LL && lload, LS && lstore, const float2 * tw,
+                                      int lane) const
+  {
+    StockhamPass<1024, 16, 1, INV>(first, lstore, tw, lane);
+    __syncthreads();
+    StockhamPassReg<1024, 16, 16, INV>(lload, lstore, a, lane);
+    __syncthreads();
+    StockhamPassReg<1024, 4, 256, INV>(lload, last, b, lane);
+  }
+};
+
+// ramp_wave_kernel for a real ramp kernel (K[k] real, as for every window of
+// include/rtkFFTRampImageFilter.hxx: the spatial kernel is even) and M <= 1024, as
+// persistent wavefronts: each wavefront filters rows blockIdx.x, blockIdx.x +
+// gridDim.x, ... with every table value it needs loaded once into registers (the
+// FFT twiddles, and for the split step exp(-2 pi i k / N) and K[k], K[M - k] of its
+// pairs k = 1 + lane + 64 i), so the rows read only the stack.  Same tables and
+// arithmetic as ramp_wave_kernel, except that the imaginary part of K (zero) is not
+// multiplied and split[M - k] = -conj(split[k]) is not read.
+// VEC (no truncation correction, nu and the zero extension multiples of 4, 16-byte
+// aligned rows): four pixels per 16-byte load, two outputs per store.
+template <typename TO, int M, bool TRUNC, bool VEC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void ramp_rows_kernel(const float * __restrict__ in, TO * __restrict__ out, int nu,
+                                                        int nv, long nrows, int zeroext,
+                                                        const float2 * __restrict__ tables, int next,
+                                                        const float * __restrict__ mirror,
+                                                        const WeightView * __restrict__ wviews, float dx, float dy)
+{
+  __shared__ float4 buf4[M / 2];
+  __shared__ float  kre[M + 1]; // the kernel, real
+  float2 *          buf = reinterpret_cast<float2 *>(buf4);
+  const int         lane = threadIdx.x;
+  const float2 *    tw = tables;
+  const float2 *    split = tables + M;
+  const float2 *    K = split + (M + 1);
+  constexpr int     SPL = (M / 2) / 64; // split pairs per lane
+  WaveTwiddles<M>   T;
+  T.load(tw, lane);
+  for (int k = lane; k <= M; k += 64)
+    kre[k] = K[k].x;
+  // split[1 + lane + 64 it] = split[1 + lane] exp(-2 pi i 64 it / N), N = 2M: the
+  // second factor is split[64 it], the same on every lane
+  const float2 sw0 = split[1 + lane];
+  float2       swf[SPL];
+#pragma unroll
+  for (int it = 0; it < SPL; it++)
+    swf[it] = split[64 * it];
+  const float scale = 1.f / float(M);
+
+  for (long rowIdx = blockIdx.x; rowIdx < nrows; rowIdx += gridDim.x)
+  {
+    const int     p = int(rowIdx / nv), j = int(rowIdx - long(p) * nv);
+    const float * row = in + rowIdx * nu;
+    TO *          orow = out + rowIdx * nu;
+    const WeightView * wv = wviews ? wviews + p : nullptr;
+    float wx0 = 0.f, wsddtana = 0.f, wnum = 0.f, wsina = 0.f, wfac = 1.f, wC = 0.f;
+    bool  wcos = false;
+    if (wv)
+    {
+      wfac = wv->factor;
+      if (wv->parallel == 0.f)
+      {
+        wcos = true;
+        const float py = wv->y0 + dy * float(j);
+        wx0 = wv->x0;
+        wsddtana = wv->sddtana;
+        wnum = wv->numpart1;
+        wsina = wv->sina;
+        wC = wv->sdd2 + py * py;
+      }
+    }
+    auto W = [&](int i) -> float {
+      if (!wcos)
+        return row[i] * wfac;
+      const float px = wx0 + dx * float(i), t = px - wsddtana;
+      return row[i] * (wfac * (wnum - px * wsina) * __builtin_amdgcn_rsqf(wC + t * t));
+    };
+    auto X = [&](int n) -> float {
+      const int i = n - zeroext;
+      if (!TRUNC)
+        return unsigned(i) < unsigned(nu) ? W(min(max(i, 0), nu - 1)) : 0.f;
+      if (i >= 0 && i < nu)
+        return W(i);
+      {
+        if (i < 0 && -i <= next)
+          return mirror[-i] * (2.f * W(1) - W(-i));
+        if (i >= nu && i - (nu - 1) <= next)
+        {
+          const int b = i - (nu - 1);
+          return mirror[b] * (2.f * W(nu - 1) - W(nu - 1 - b));
+        }
+      }
+      return 0.f;
+    };
+    // the previous row's last LDS reads are done before its buffer is overwritten
+    __syncthreads();
+    if constexpr (VEC)
+    {
+      for (int q = lane; q < M / 2; q += 64)
+      {
+        const int i = 4 * q - zeroext; // a chunk lies wholly inside the row or outside it
+        float4    v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (unsigned(i) < unsigned(nu))
+        {
+          v = *reinterpret_cast<const float4 *>(row + i);
+          if (wcos)
+          {
+            auto wt = [&](int e) {
+              const float px = wx0 + dx * float(i + e), t = px - wsddtana;
+              return wfac * (wnum - px * wsina) * __builtin_amdgcn_rsqf(wC + t * t);
+            };
+            v = make_float4(v.x * wt(0), v.y * wt(1), v.z * wt(2), v.w * wt(3));
+          }
+          else
+            v = make_float4(v.x * wfac, v.y * wfac, v.z * wfac, v.w * wfac);
+        }
+        buf4[q] = v;
+      }
+    }
+    else
+      for (int q = lane; q < M / 2; q += 64)
+        buf4[q] = make_float4(X(4 * q), X(4 * q + 1), X(4 * q + 2), X(4 * q + 3));
+    __syncthreads();
+    auto lload = [&](int m) { return buf[m]; };
+    auto lstore = [&](int m, float2 v) { buf[m] = v; };
+    T.template fft<false>(lload, lstore, lload, lstore, tw, lane);
+    __syncthreads();
+
+    // split to the half spectrum, multiply by the kernel and re-pack, in place: lane
+    // pairs (k, M - k) for k = 1 + lane + 64 it; k = 0 by lane 0 (split[0] = 1)
+    auto Xof = [](float2 A, float2 Bv, float2 w) {
+      const float2 e = make_float2(0.5f * (A.x + Bv.x), 0.5f * (A.y - Bv.y));
+      const float2 o = make_float2(0.5f * (A.y + Bv.y), -0.5f * (A.x - Bv.x));
+      const float2 wo = cmul(w, o);
+      return make_float2(e.x + wo.x, e.y + wo.y);
+    };
+    auto Zof = [](float2 A, float2 Bv, float2 w) {
+      const float2 e = make_float2(0.5f * (A.x + Bv.x), 0.5f * (A.y - Bv.y));
+      const float2 d = make_float2(0.5f * (A.x - Bv.x), 0.5f * (A.y + Bv.y));
+      const float2 o = cmul(d, conj2(w));
+      return make_float2(e.x - o.y, e.y + o.x);
+    };
+#pragma unroll
+    for (int it = 0; it < SPL; it++)
+    {
+      const int    k = 1 + lane + 64 * it, mk = M - k;
+      const float2 Zk = buf[k], Zm = buf[mk];
+      const float2 s = it == 0 ? sw0 : cmul(sw0, swf[it]);
+      const float2 sm = make_float2(-s.x, s.y); // split[M - k] = -conj(split[k])
+      const float2 Xk = Xof(Zk, Zm, s), Xm = Xof(Zm, Zk, sm);
+      const float  kk = kre[k], km = kre[mk];
+      const float2 Yk = make_float2(Xk.x * kk, Xk.y * kk);
+      const float2 Ym = make_float2(Xm.x * km, Xm.y * km);
+      buf[k] = Zof(Yk, Ym, s);
+      if (mk != k)
+        buf[mk] = Zof(Ym, Yk, sm);
+    }
+    if (lane == 0)
+    {
+      const float2 Z0 = buf[0];
+      const float2 Y0 = make_float2((Z0.x + Z0.y) * kre[0], 0.f), YM = make_float2((Z0.x - Z0.y) * kre[M], 0.f);
+      buf[0] = Zof(Y0, YM, make_float2(1.f, 0.f));
+    }
+    __syncthreads();
+    auto gstore = [&](int m, float2 z) {
+      const int i0 = 2 * m - zeroext;
+      if constexpr (VEC)
+      {
+        if (unsigned(i0) < unsigned(nu)) // i0 even, nu even: both outputs or neither
+          StoreOut2(orow, i0, z.x * scale, z.y * scale);
+      }
+      else
+      {
+        if (i0 >= 0 && i0 < nu)
+          StoreOut(orow, i0, z.x * scale);
+        if (i0 + 1 >= 0 && i0 + 1 < nu)
+          StoreOut(orow, i0 + 1, z.y * scale);
+      }
+    };
+    T.template fft<true>(lload, gstore, lload, lstore, tw, lane);
+  }
 }
 
 // ------------------------------------------------------- mixed-radix FFTs --
@@ -962,6 +1291,65 @@ rtkhip_ramp_filter(const int proj_dim[3], int xpad, const double * kernel, int t
       auto & dwt = g_ramp.waveTables[s];
       dwt.Resize(wt.size());
       RTKHIP_CHECK(hipMemcpyAsync(dwt.get(), wt.data(), sizeof(float2) * wt.size(), hipMemcpyHostToDevice, s));
+      // a real kernel spectrum (the spatial kernel is even: every window of the ramp
+      // filter) and M <= 1024: persistent wavefronts with their tables in registers
+      double kre = 0., kim = 0.;
+      for (int k = 0; k <= M; k++)
+      {
+        kre = std::max(kre, std::abs(kernel[2 * k]));
+        kim = std::max(kim, std::abs(kernel[2 * k + 1]));
+      }
+      if (M <= 1024 && kim <= 1e-9 * kre)
+      {
+        const long nrows = long(nv) * np;
+        int        dev = 0, cus = 0, perCu = 0;
+        RTKHIP_CHECK(hipGetDevice(&dev));
+        RTKHIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        // rows of 16-byte aligned 4-pixel chunks, wholly inside the row or in the padding
+        const bool vec = trunc_next == 0 && nu % 4 == 0 && zeroext % 4 == 0 &&
+                         reinterpret_cast<uintptr_t>(dev_in) % 16 == 0 &&
+                         reinterpret_cast<uintptr_t>(dev_out) % 16 == 0;
+#define RTKHIP_RAMPR(TO, MM, TR, VE)                                                                                 \
+  do                                                                                                                 \
+  {                                                                                                                  \
+    RTKHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, ramp_rows_kernel<TO, MM, TR, VE>, 64, 0));     \
+    const long g = std::min<long>(nrows, long(std::max(cus, 1)) * std::max(perCu, 1));                               \
+    hipLaunchKernelGGL((ramp_rows_kernel<TO, MM, TR, VE>), dim3(unsigned(g)), dim3(64), 0, s, dev_in,                 \
+                       static_cast<TO *>(dev_out), nu, nv, nrows, int(zeroext), dwt.get(), trunc_next, dmirror, dw,   \
+                       dx, dy);                                                                                      \
+  } while (0)
+#define RTKHIP_RAMPR_T(TO, MM)                                                                                       \
+  do                                                                                                                 \
+  {                                                                                                                  \
+    if (trunc_next > 0)                                                                                              \
+      RTKHIP_RAMPR(TO, MM, true, false);                                                                             \
+    else if (vec)                                                                                                    \
+      RTKHIP_RAMPR(TO, MM, false, true);                                                                             \
+    else                                                                                                             \
+      RTKHIP_RAMPR(TO, MM, false, false);                                                                            \
+  } while (0)
+#define RTKHIP_RAMPR_M(TO)                                                                                           \
+  do                                                                                                                 \
+  {                                                                                                                  \
+    if (M == 256)                                                                                                    \
+      RTKHIP_RAMPR_T(TO, 256);                                                                                       \
+    else if (M == 512)                                                                                               \
+      RTKHIP_RAMPR_T(TO, 512);                                                                                       \
+    else                                                                                                             \
+      RTKHIP_RAMPR_T(TO, 1024);                                                                                      \
+  } while (0)
+        if (out_dtype == RTKHIP_F32)
+          RTKHIP_RAMPR_M(float);
+        else if (out_dtype == RTKHIP_F16)
+          RTKHIP_RAMPR_M(__half);
+        else
+          RTKHIP_RAMPR_M(__hip_bfloat16);
+#undef RTKHIP_RAMPR_M
+#undef RTKHIP_RAMPR_T
+#undef RTKHIP_RAMPR
+        RTKHIP_CHECK_LAUNCH();
+        return;
+      }
       dim3 wgrid(nv, np);
 #define RTKHIP_RAMPW(TO, MM)                                                                                         \
   do                                                                                                                 \
