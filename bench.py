@@ -185,6 +185,31 @@ def projector_pair_measurement(rtk, np, torch, g, q, proj, vol, reps=2):
     return out
 
 
+def ramp_measurement(rtk, np, torch, proj, reps=5):
+    """The ramp filter alone (Hann window 0.8, zero padding to 2048 columns) of the
+    720 x 1024^2 stack: the stack in and the filtered stack out.  Kernel time from HIP
+    events on the launch stream, best of `reps`."""
+    f = rtk.HipFFTRampImageFilter()
+    f.SetInput(0, proj)
+    f.SetHannCutFrequency(0.8)
+    f.Update()
+    nbytes = proj.tensor.numel() * 4 * 2
+    ms = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        f.Update()
+        e1.record()
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    t = min(ms)
+    out = {"workload": "ramp filter of the 720 x 1024^2 FP32 stack (Hann 0.8, padded to 2048)",
+           "ms": round(t, 3), "bytes": int(nbytes), "gb_per_s": round(nbytes / t / 1e6, 1),
+           "frac_of_8tb": round(nbytes / t / 1e6 / 8000., 3)}
+    log(f"ramp filter: {out}")
+    return out
+
+
 def weighting_measurement(rtk, np, torch, proj, reps=3):
     """Parker short-scan and displaced-detector weighting of the 720 x 1024^2 stack
     (HBM streams): a 240 degree arc for Parker (stack in, stack out), a detector
@@ -456,6 +481,7 @@ def main():
     ap.add_argument("--cg-only", action="store_true", help="skip the FDK measurement")
     ap.add_argument("--joseph-only", action="store_true",
                     help="only the Joseph FP / BP pair at the sparse-view CG shape")
+    ap.add_argument("--ramp-only", action="store_true", help="only the ramp filter of the FDK stack")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: rtkhip has no multi-GPU path "
@@ -489,6 +515,9 @@ def main():
     log("building workload (analytic projections of the 20-ellipsoid phantom)")
     rows, g, q, proj, vol = build_workload(rtk, np)
     torch.cuda.synchronize()
+    if args.ramp_only:
+        print(json.dumps({"ramp_filter": ramp_measurement(rtk, np, torch, proj)}), flush=True)
+        return
 
     fdk = rtk.HipFDKConeBeamReconstructionFilter()
     fdk.SetInput(0, vol)
@@ -562,6 +591,7 @@ def main():
     if not args.no_pair:
         result["projector_pair"] = projector_pair_measurement(rtk, np, torch, g, q, proj, vol)
     if not args.no_weights:
+        result["ramp_filter"] = ramp_measurement(rtk, np, torch, proj)
         result["short_scan_weighting"] = weighting_measurement(rtk, np, torch, proj)
     if not args.no_cg:
         result["joseph_pair_sparse_view"] = joseph_pair_measurement(rtk, np, torch)
