@@ -1324,6 +1324,233 @@ __global__ __launch_bounds__(256) void joseph_gather_shell_kernel(
   vol[idx] += acc;
 }
 
+// Forward projection as a gather, for the geometries the gather backprojection takes
+// (detector columns vertical in the volume).  A column's rays lie in one vertical
+// plane: at slice k of their dominant axis they share the horizontal position X(k),
+// and a middle sample's bilinear interpolation (.hxx:182-543) is the linear
+// interpolation, at the ray's height y'(k), of the profile g(y) = (1 - fx) v(hx, y) +
+// fx v(hx + 1, y) with hx = floor(X), fx = X - hx, the same for every ray of the
+// column.  A workgroup takes 16 columns x kFRows rows of one view; per slice its
+// threads build the 16 profiles over the rows' height range in LDS (double-buffered,
+// one barrier per slice), then each thread adds the middle samples of its kFRays rays
+// (rows 16 apart) from two LDS reads each.  First and last samples, at the ray's own (clamped)
+// positions, read their four taps from global memory as the per-ray projector does.
+// Rays of columns that mix dominant axes are walked per ray.
+constexpr int kFRows = 64; // detector rows per workgroup
+constexpr int kFRays = kFRows / 16; // rays per thread (rows 16 apart)
+constexpr int kFProf = 48; // profile entries per column: kFRows |b| + 3 with |b| < 2/3
+constexpr int kFPad = 49;  // LDS stride of a column's profile (its columns on distinct banks)
+
+// The first and last samples, and every sample of a ray walked alone (a column that
+// mixes dominant axes), as the per-ray forward projector computes them.
+__device__ __forceinline__ float
+RayBorder(const Ray & R, double step, const float * bb, double x, double y)
+{
+  int   tt[4];
+  float w[4];
+  BorderTaps(R, x, y, tt, w);
+  float r = 0.f;
+  r += w[0] * bb[tt[0]];
+  r += w[1] * bb[tt[1]];
+  r += w[2] * bb[tt[2]];
+  r += w[3] * bb[tt[3]];
+  return float(double(r) * step);
+}
+
+__device__ __forceinline__ float
+RayEnds(const Ray & R, const float * vol, const float * volT)
+{
+  const float * b = (R.transposed ? volT : vol) + R.start;
+  if (R.fs == R.ns)
+    return RayBorder(R, R.wOne, b, R.cx, R.cy);
+  const float first = RayBorder(R, R.wB, b, R.cx, R.cy);
+  const double n = double(R.nmid + 1);
+  return first + RayBorder(R, R.wE, b + (R.nmid + 1) * R.oz, R.cx + n * R.stepx, R.cy + n * R.stepy);
+}
+
+__device__ __forceinline__ float
+RayWalk(const Ray & R, const float * vol, const float * volT)
+{
+  const float * b = (R.transposed ? volT : vol) + R.start;
+  if (R.fs == R.ns)
+    return RayBorder(R, R.wOne, b, R.cx, R.cy);
+  float sum = RayBorder(R, R.wB, b, R.cx, R.cy);
+  b += R.oz;
+  Walker wk;
+  wk.Init(R.cx + R.stepx, R.cy + R.stepy);
+  for (int s2 = 0; s2 < R.nmid; ++s2)
+  {
+    float       lx, ly;
+    const int   idx = wk.Stage(R, lx, ly);
+    const float lxc = 1.f - lx, lyc = 1.f - ly;
+    sum += (lxc * lyc) * b[idx] + (lx * lyc) * b[idx + 1] + (lxc * ly) * b[idx + R.oy] + (lx * ly) * b[idx + 1 + R.oy];
+    b += R.oz;
+  }
+  return sum + RayBorder(R, R.wE, b, wk.PosX(), wk.PosY());
+}
+
+// Middle samples of the columns of dominant axis A (0: slices along x in the z-fastest
+// copy, horizontal z; 2: slices along z, horizontal x), slices klo..khi (uniform over
+// the workgroup).  Ray q of this thread takes the slices lo[q]..hi[q].  The profile
+// values of slice k + 1 are loaded while the rays of slice k are summed.
+template <int A>
+__device__ __forceinline__ void
+ForwardGatherSweep(const GatherView & V, const GatherColumn & C, bool colOn, const int (&lo)[kFRays], const int (&hi)[kFRays],
+                   int jv0, int rb, int rlast, const float * __restrict__ buf, int nsl, int nh, int ny, int klo,
+                   int khi, float (*prof)[16][kFPad], float (&acc)[kFRays])
+{
+  const int    t = threadIdx.x, c = t & 15, rq = t >> 4;
+  const size_t sk = size_t(ny) * nh; // slice stride; rows nh apart, the horizontal axis unit-stride
+  const double ck = 0.5 * (nsl - 1), ch = 0.5 * (nh - 1), cy = 0.5 * (ny - 1);
+  constexpr int NM = kFProf / 16; // profile entries per thread
+  // the column at slice k: the rows' heights a + b jv relative to the profile's first
+  // height ylo (Af + Bf jv), and this thread's profile entries as tap pairs
+  struct Slice
+  {
+    float  Af, Bf, fx;
+    int    ylo;
+    float2 v[NM];
+  };
+  // X and t advance by h1 and t1 per slice; a and b are affine in t
+  const double kd0 = double(klo) - ck;
+  double       X = C.h0 + kd0 * C.h1 + ch, tt = C.t0 + kd0 * C.t1;
+  const double dsy = V.sy - V.y0, a0 = V.y0 + cy;
+  auto         fetch = [&](int k, Slice & S) {
+    if (!colOn || k > khi)
+      return;
+    const double a = fma(tt, dsy, a0), b = fma(-V.y1, tt, V.y1);
+    const double fl = floor(X);
+    const int    hx = min(max(int(fl), 0), nh - 2);
+    S.fx = float(X - fl);
+    S.ylo = int(floor(fmin(a + b * rb, a + b * rlast)));
+    S.Af = float(a - double(S.ylo));
+    S.Bf = float(b);
+    // entries rq + 16 m: heights ylo + rq + 16 m (zero outside the volume)
+    const int     y0 = S.ylo + rq;
+    const float * q0 = buf + (size_t(k) * sk + hx) + ptrdiff_t(y0) * nh;
+#pragma unroll
+    for (int m = 0; m < NM; m++)
+    {
+      const float * q = q0 + 16 * m * nh;
+      S.v[m] = unsigned(y0 + 16 * m) < unsigned(ny) ? make_float2(q[0], q[1]) : make_float2(0.f, 0.f);
+    }
+    X += C.h1;
+    tt += C.t1;
+  };
+  Slice cur{}, nxt{};
+  fetch(klo, cur);
+  for (int k = klo; k <= khi; k++)
+  {
+    float(*g)[kFPad] = prof[k & 1];
+    if (colOn)
+    {
+#pragma unroll
+      for (int m = 0; m < NM; m++)
+        g[c][rq + 16 * m] = (1.f - cur.fx) * cur.v[m].x + cur.fx * cur.v[m].y;
+    }
+    __syncthreads();
+    fetch(k + 1, nxt);
+    if (colOn)
+    {
+#pragma unroll
+      for (int q = 0; q < kFRays; q++)
+      {
+        if (k < lo[q] || k > hi[q])
+          continue;
+        // the row pair of the ray's height, its index clamped as the per-ray walk
+        // clamps it, the fraction unclamped
+        // (entries outside the volume are zero; the index bound only guards rounding)
+        const float u = fmaxf(fmaf(cur.Bf, float(jv0 + 16 * q), cur.Af), 0.f); // y' - ylo (>= 0 but for rounding)
+        const float fu = floorf(u);
+        const float fy = u - fu;
+        const int   iy = min(int(fu), kFProf - 2);
+        const float g0 = g[c][iy];
+        acc[q] += fmaf(fy, g[c][iy + 1] - g0, g0);
+      }
+    }
+    cur = nxt;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void joseph_fp_gather_kernel(
+  const JosephView * __restrict__ views, const GatherView * __restrict__ gviews,
+  const GatherColumn * __restrict__ gcols, const signed char * __restrict__ colAxis, int nu, int nv,
+  const float * pin, float * pout, const float * __restrict__ vol, const float * __restrict__ volT, int nx, int ny,
+  int nz, double spx, double spy, double spz, double infClip, double supClip)
+{
+  __shared__ float prof[2][16][kFPad];
+  __shared__ int   krange[2][2]; // [x / z columns][first, last middle slice]
+  const int        t = threadIdx.x, c = t & 15, rq = t >> 4;
+  const int        p = blockIdx.z, iu = blockIdx.x * 16 + c, rb = blockIdx.y * kFRows;
+  const int        rlast = min(rb + kFRows, nv) - 1;
+  const int        jv0 = rb + rq; // this thread's rays: rows jv0 + 16 q
+  if (t < 2)
+  {
+    krange[t][0] = INT_MAX;
+    krange[t][1] = INT_MIN;
+  }
+  __syncthreads();
+  const int cax = iu < nu ? int(colAxis[size_t(p) * nu + iu]) : -1;
+  float     acc[kFRays];
+  double    norm[kFRays];
+  int       lo[kFRays], hi[kFRays];
+  bool      hit[kFRays];
+  int       klo = INT_MAX, khi = INT_MIN;
+#pragma unroll
+  for (int q = 0; q < kFRays; q++)
+  {
+    const int jv = jv0 + 16 * q;
+    Ray       R;
+    acc[q] = 0.f;
+    norm[q] = 0.;
+    hit[q] = iu < nu && jv < nv && SetupRay<false>(views[p], iu, jv, p, nx, ny, nz, spx, spy, spz, infClip, supClip, R);
+    lo[q] = 1;
+    hi[q] = 0;
+    if (!hit[q])
+      continue;
+    norm[q] = R.stepNorm;
+    if (cax < 0 || R.axis != cax)
+      acc[q] = RayWalk(R, vol, volT); // a column that mixes dominant axes
+    else
+    {
+      acc[q] = RayEnds(R, vol, volT);
+      lo[q] = min(R.ns, R.fs) + 1; // middle slices
+      hi[q] = max(R.ns, R.fs) - 1;
+      klo = min(klo, lo[q]);
+      khi = max(khi, hi[q]);
+    }
+  }
+  if (cax >= 0 && klo <= khi)
+  {
+    atomicMin(&krange[cax >> 1][0], klo);
+    atomicMax(&krange[cax >> 1][1], khi);
+  }
+  __syncthreads();
+  const GatherView V = gviews[p];
+  GatherColumn     C{};
+  if (cax >= 0)
+    C = gcols[size_t(p) * nu + iu];
+  if (krange[0][0] <= krange[0][1])
+    ForwardGatherSweep<0>(V, C, cax == 0, lo, hi, jv0, rb, rlast, volT, nx, nz, ny, krange[0][0], krange[0][1], prof,
+                          acc);
+  if (krange[1][0] <= krange[1][1])
+    ForwardGatherSweep<2>(V, C, cax == 2, lo, hi, jv0, rb, rlast, vol, nz, nx, ny, krange[1][0], krange[1][1], prof,
+                          acc);
+#pragma unroll
+  for (int q = 0; q < kFRays; q++)
+  {
+    const int jv = jv0 + 16 * q;
+    if (iu >= nu || jv >= nv)
+      continue;
+    const long pix = (long(p) * nv + jv) * nu + iu;
+    if (hit[q])
+      pout[pix] = float(double(pin ? pin[pix] : 0.f) + double(acc[q]) * norm[q]);
+    else
+      pout[pix] = pin ? pin[pix] : 0.f;
+  }
+}
+
 struct JosephCache
 {
   std::map<hipStream_t, DeviceBuffer<JosephView>>          views;
@@ -1346,6 +1573,7 @@ struct JosephCache
   std::map<hipStream_t, std::vector<double>> gplanKey;
   std::map<hipStream_t, long long>           gplanBlocks; // -1: the gather does not apply
   std::map<hipStream_t, int>                 gplanFree;
+  std::map<hipStream_t, bool>                gplanForward;
 };
 thread_local JosephCache g_joseph;
 
@@ -1357,6 +1585,7 @@ struct GatherPlan
   std::vector<signed char>  axis;
   std::vector<int>          blocks; // packed block coordinates (bx | bz << 10 | by << 20)
   int                       freeRows = 1; // leading candidate rows whose weight needs no clamp
+  bool                      forward = false; // the forward gather's profiles hold kFRows rows
 };
 
 // True when the geometry lets the gather take the backprojection: default clips,
@@ -1477,6 +1706,9 @@ PlanGather(const int proj_dim[3], const int vol_dim[3], const double * proj_to_v
   // the first candidate row has |y' - j| < 1 and rows advance by the step b, so rows
   // 0..2 stay inside the window when 3 |b| < 2 (a margin covers the FP32 arithmetic)
   G.freeRows = maxRowStep < (2. / 3.) * (1. - 1e-4) ? 3 : 1;
+  // kFRows rows span (kFRows - 1) |b| + 1 heights above the profile's first one, and
+  // a middle sample reads two entries
+  G.forward = maxRowStep * (kFRows - 1) * (1. + 1e-4) + 3. <= double(kFProf);
   const int nbx = (nx + 15) / 16, nbz = (nz + 15) / 16, nby = (ny + kGNJ - 1) / kGNJ;
   G.blocks.clear();
   for (int by = 0; by < nby; by++)
@@ -1495,7 +1727,7 @@ PlanGather(const int proj_dim[3], const int vol_dim[3], const double * proj_to_v
 // of gather blocks, or -1 when the gather does not apply.
 long long
 PrepareGather(const int proj_dim[3], const int vol_dim[3], const double * proj_to_vol, const double * source,
-              double inferior_clip, double superior_clip, hipStream_t s, int & freeRows)
+              double inferior_clip, double superior_clip, hipStream_t s, int & freeRows, bool * forward = nullptr)
 {
   const int           np = proj_dim[2], nu = proj_dim[0];
   std::vector<double> key = { double(proj_dim[0]), double(proj_dim[1]), double(np),           double(vol_dim[0]),
@@ -1509,6 +1741,8 @@ PrepareGather(const int proj_dim[3], const int vol_dim[3], const double * proj_t
   if (cached == key)
   {
     freeRows = g_joseph.gplanFree[s];
+    if (forward)
+      *forward = g_joseph.gplanForward[s];
     return g_joseph.gplanBlocks[s];
   }
   cached.clear(); // a failure below leaves no stale entry
@@ -1534,6 +1768,9 @@ PrepareGather(const int proj_dim[3], const int vol_dim[3], const double * proj_t
   }
   g_joseph.gplanBlocks[s] = nblocks;
   g_joseph.gplanFree[s] = freeRows = G.freeRows;
+  g_joseph.gplanForward[s] = nblocks >= 0 && G.forward;
+  if (forward)
+    *forward = g_joseph.gplanForward[s];
   cached = std::move(key);
   return nblocks;
 }
@@ -1699,9 +1936,24 @@ rtkhip_joseph_forward_project(const int proj_dim[3], const int vol_dim[3], const
     }
     else
     {
-      hipLaunchKernelGGL(joseph_kernel<false>, grid, dim3(256), 0, s, dv, nu, nv, dev_proj_in, dev_proj_out, dev_vol,
-                         dT.get(), vol_dim[0], vol_dim[1], vol_dim[2], vol_spacing[0], vol_spacing[1], vol_spacing[2],
-                         infClip, supClip, nullptr);
+      // detector columns vertical in the volume: the gather (same tables as the gather
+      // backprojection, kept across calls with the same geometry)
+      int       freeRows = 1;
+      bool      forward = false;
+      const int nblocks =
+        int(PrepareGather(proj_dim, vol_dim, proj_to_vol, source, inferior_clip, superior_clip, s, freeRows, &forward));
+      if (nblocks >= 0 && forward)
+      {
+        dim3 fg((nu + 15) / 16, (nv + kFRows - 1) / kFRows, np);
+        hipLaunchKernelGGL(joseph_fp_gather_kernel, fg, dim3(256), 0, s, dv, g_joseph.gviews[s].get(),
+                           g_joseph.gcols[s].get(), g_joseph.gaxis[s].get(), nu, nv, dev_proj_in, dev_proj_out,
+                           dev_vol, dT.get(), vol_dim[0], vol_dim[1], vol_dim[2], vol_spacing[0], vol_spacing[1],
+                           vol_spacing[2], infClip, supClip);
+      }
+      else
+        hipLaunchKernelGGL(joseph_kernel<false>, grid, dim3(256), 0, s, dv, nu, nv, dev_proj_in, dev_proj_out,
+                           dev_vol, dT.get(), vol_dim[0], vol_dim[1], vol_dim[2], vol_spacing[0], vol_spacing[1],
+                           vol_spacing[2], infClip, supClip, nullptr);
       RTKHIP_CHECK_LAUNCH();
     }
   });
