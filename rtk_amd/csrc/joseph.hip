@@ -1482,7 +1482,10 @@ __global__ __launch_bounds__(256) void joseph_fp_gather_kernel(
   __shared__ float prof[2][16][kFPad];
   __shared__ int   krange[2][2]; // [x / z columns][first, last middle slice]
   const int        t = threadIdx.x, c = t & 15, rq = t >> 4;
-  const int        p = blockIdx.z, iu = blockIdx.x * 16 + c, rb = blockIdx.y * kFRows;
+  // grid (column blocks, views, row bands), as the per-ray projector: workgroups in
+  // flight share a band of detector rows, whose rays stay in one slab of the volume,
+  // so the slab stays in L2 while every view reads it
+  const int        p = blockIdx.y, iu = blockIdx.x * 16 + c, rb = blockIdx.z * kFRows;
   const int        rlast = min(rb + kFRows, nv) - 1;
   const int        jv0 = rb + rq; // this thread's rays: rows jv0 + 16 q
   if (t < 2)
@@ -1944,7 +1947,7 @@ rtkhip_joseph_forward_project(const int proj_dim[3], const int vol_dim[3], const
         int(PrepareGather(proj_dim, vol_dim, proj_to_vol, source, inferior_clip, superior_clip, s, freeRows, &forward));
       if (nblocks >= 0 && forward)
       {
-        dim3 fg((nu + 15) / 16, (nv + kFRows - 1) / kFRows, np);
+        dim3 fg((nu + 15) / 16, np, (nv + kFRows - 1) / kFRows);
         hipLaunchKernelGGL(joseph_fp_gather_kernel, fg, dim3(256), 0, s, dv, g_joseph.gviews[s].get(),
                            g_joseph.gcols[s].get(), g_joseph.gaxis[s].get(), nu, nv, dev_proj_in, dev_proj_out,
                            dev_vol, dT.get(), vol_dim[0], vol_dim[1], vol_dim[2], vol_spacing[0], vol_spacing[1],
