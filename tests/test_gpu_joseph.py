@@ -98,3 +98,75 @@ def test_joseph_constant_volume_chords_large(oracle, hip):
         chord = np.maximum(0., t1 - t0) * np.linalg.norm(d, axis=-1)
         assert np.max(np.abs(got[k] - chord)) < 1e-4 * 256, k
         assert rel_l2(got[k], chord) < 1e-6
+
+
+# A geometry the column gathers take (detector columns vertical in the volume, default
+# clips, row step b = 0.9 mm x (distance from the source / 800 mm) between 0.51 and 0.62
+# voxel over the box, inside the forward gather's (0.71) and the backprojection gather's
+# (0.4 to 2/3) bounds), with detector and source offsets, views on the diagonals (columns
+# whose rays switch dominant axis), and sizes that are not multiples of the workgroup
+# tiles (16 columns, 64 rows).
+GATHER_ROWS = np.array([[500., 800., 7. + 15. * i, 2.5 - 0.4 * i, -3. + 0.3 * i, 0., 0., 1.5, -2. + 0.2 * i]
+                        for i in range(24)])
+GATHER_VOL = dict(size=(64, 48, 56), spacing=(1., 1., 1.), origin=(-31., -24.5, -27.))
+GATHER_DET = dict(size=(120, 100), spacing=(0.9, 0.9), origin=(-54.5, -44.))
+
+
+def _gather_case(oracle, hip, seed):
+    rows, v, d = GATHER_ROWS, GATHER_VOL, GATHER_DET
+    n = len(rows)
+    rng = np.random.default_rng(seed)
+    vol = rng.random(v["size"][::-1]).astype(np.float32)
+    p0 = rng.normal(size=(n, d["size"][1], d["size"][0])).astype(np.float32)
+    ov = oracle.image(v["size"], v["origin"], v["spacing"])
+    os_ = oracle.image(d["size"] + (n,), d["origin"] + (0.,), d["spacing"] + (1.,))
+    return rows, v, d, vol, p0, ov, os_
+
+
+def test_joseph_gather_forward_parity(oracle, hip):
+    """The forward gather (a call that does not count samples) against the float and
+    double oracles, with a non-zero input stack, and against the per-ray projector (the
+    call that counts samples) on the same inputs."""
+    rows, v, d, vol, p0, ov, os_ = _gather_case(oracle, hip, 31)
+    ref = oracle.joseph(rows, os_, p0, ov, vol)
+    ref_d = oracle.joseph(rows, os_, p0.astype(np.float64), ov, vol.astype(np.float64))
+    g = hip_geometry(hip, rows)
+    outs = []
+    for count in (False, True):
+        f = hip.HipForwardProjectionImageFilter()
+        f.SetInput(0, to_device(hip, p0, d["origin"] + (0.,), d["spacing"] + (1.,)))
+        f.SetInput(1, to_device(hip, vol, v["origin"], v["spacing"]))
+        f.SetGeometry(g)
+        f.Update(count_samples=count)
+        outs.append(f.GetOutput().numpy())
+    gather, per_ray = outs
+    assert rel_l2(gather - p0, ref - p0) <= TOL
+    assert rel_l2(gather - p0, ref_d - p0) <= TOL
+    assert rel_l2(gather - p0, per_ray - p0) <= TOL
+    # every view, including the diagonal ones
+    for k in range(len(rows)):
+        assert rel_l2(gather[k] - p0[k], ref[k] - p0[k]) <= 3 * TOL, k
+
+
+def test_joseph_gather_backprojection_and_adjoint(oracle, hip):
+    """The backprojection gather on the same geometry against the oracle, and the
+    adjoint identity <FP x, p> = <x, BP p> between the two gathers."""
+    rows, v, d, vol, p0, ov, os_ = _gather_case(oracle, hip, 32)
+    zero = np.zeros(v["size"][::-1], np.float32)
+    ref = oracle.joseph_back(rows, ov, zero, os_, p0)
+    g = hip_geometry(hip, rows)
+    bp = hip.HipJosephBackProjectionImageFilter()
+    bp.SetInput(0, to_device(hip, zero, v["origin"], v["spacing"]))
+    bp.SetInput(1, to_device(hip, p0, d["origin"] + (0.,), d["spacing"] + (1.,)))
+    bp.SetGeometry(g)
+    bp.Update()
+    back = bp.GetOutput().numpy()
+    assert rel_l2(back, ref) <= TOL
+    fp = hip.HipForwardProjectionImageFilter()
+    fp.SetInput(0, to_device(hip, np.zeros_like(p0), d["origin"] + (0.,), d["spacing"] + (1.,)))
+    fp.SetInput(1, to_device(hip, vol, v["origin"], v["spacing"]))
+    fp.SetGeometry(g)
+    fp.Update()
+    a = float((fp.GetOutput().numpy().astype(np.float64) * p0).sum())
+    b = float((vol.astype(np.float64) * back).sum())
+    assert abs(a / b - 1) < 1e-5, (a, b)
