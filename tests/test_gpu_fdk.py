@@ -96,6 +96,49 @@ def test_fdk_weight_plus_ramp(oracle, hip, kw, nu):
     assert e <= 1e-5, f"{kw}: rel L2 {e:.3g}"
 
 
+# (width, output type): the persistent wavefront kernel with 16-byte row access (nu and the
+# zero extension multiples of 4: 200 -> 512 padded, 600 -> 2048) and without it (212:
+# zero extension 150), writing FP32, FP16 or BF16
+RAMP16_CASES = [(200, "f16"), (200, "bf16"), (212, "f16"), (600, "bf16"), (600, "f32")]
+
+
+@pytest.mark.parametrize("nu,dt", RAMP16_CASES, ids=[f"{n}-{d}" for n, d in RAMP16_CASES])
+def test_weight_plus_ramp_output_types(oracle, hip, nu, dt):
+    """Weighting + ramp written as FP32, FP16 or BF16 (the backprojector reads the
+    16-bit stacks): against the float oracle within the output type's rounding, one
+    unit of its precision in relative L2 (2^-11 for FP16, 2^-8 for BF16)."""
+    from rtk_amd import _lib
+    n, nv = 4, 7
+    rows = _weight_rows(n)
+    st = oracle.image((nu, nv, n), (-60., -6., 0.), (1.0, 1.3, 1.))
+    rng = np.random.default_rng(9)
+    p = (1 + 0.1 * rng.normal(size=(n, nv, nu))).astype(np.float32)
+    ref = oracle.ramp(st, oracle.fdk_weight(rows, st, p), gpf=2, hann=0.8)
+    xpad, _ = oracle.padded_size(nu, nv, gpf=2)
+    K = oracle.ramp_kernel(xpad, 1, 1.0, hann=0.8)[0]
+    kern = np.stack([K.real, K.imag], -1).ravel()
+    _, src, ang, gaps, tilt = oracle.geometry(rows)
+    g7 = np.zeros((n, 7))
+    for k in range(n):
+        f = gaps[k] * rows[k, 1] / (2 * rows[k, 0] ** 2) * np.linalg.norm(src[k])
+        g7[k] = [rows[k, 1], rows[k, 0], rows[k, 3], rows[k, 4], rows[k, 8], f, tilt[k]]
+    x = torch.from_numpy(p).cuda()
+    code, tdt, u = {"f32": (_lib.F32, torch.float32, 2.**-24), "f16": (_lib.F16, torch.float16, 2.**-11),
+                    "bf16": (_lib.BF16, torch.bfloat16, 2.**-8)}[dt]
+    y = torch.empty(x.shape, dtype=tdt, device="cuda")
+    kk, kp = _lib.dbl(kern)
+    m, mp = _lib.dbl(np.zeros(1))
+    a, ap = _lib.dbl(g7)
+    o, op = _lib.dbl([-60., -6.])
+    s2, sp = _lib.dbl([1.0, 1.3])
+    _lib.check(_lib.lib().rtkhip_ramp_filter(_lib.int3((nu, nv, n)), xpad, kp, 0, mp, ap, op, sp,
+                                             ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()), code,
+                                             None))
+    torch.cuda.synchronize()
+    e = rel_l2(y.float().cpu().numpy(), ref)
+    assert e <= max(u, 1e-5), f"{nu} {dt}: rel L2 {e:.3g}"
+
+
 def _fdk_case(oracle, hip, n=180):
     from rtk_amd import phantoms
     rows = np.array([[600., 1200., i * 360. / n, 0, 0, 0, 0, 20, 15] for i in range(n)])
