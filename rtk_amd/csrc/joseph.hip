@@ -1289,34 +1289,33 @@ __global__ __launch_bounds__(256) void joseph_gather_shell_kernel(
         const int          d0 = int(ceilf((-sg - base) * rb));
         const float        wh = fmaxf(0.f, 1.f - fabsf(float(X) - hc));
         const float        whc = fmaxf(0.f, 1.f - fabsf(fminf(fmaxf(float(X), -hmax), hmax) - hc));
+        // the rows' slices and values, all loads issued together (rows outside the
+        // detector read a clamped row and take weight zero; a ray the gather does not
+        // take has value zero and weights zero)
+        const size_t colBase = size_t(p) * viewRays + iu;
+        uint2        ve[kGRows];
+#pragma unroll
+        for (int r = 0; r < kGRows; r++)
+          ve[r] = gend[colBase + size_t(min(max(jref + d0 + r, 0), nv - 1)) * nu];
+#pragma unroll
         for (int r = 0; r < kGRows; r++)
         {
           const int jv = jref + d0 + r;
-          if (jv < 0 || jv >= nv)
-            continue;
-          // the middle-step and end-step weights of this row; no read when both are zero
+          // the middle-step and end-step weights of this row
           const float yabs = fmaf(bf, float(d0 + r), base) + yv;
           const float wm = wh * fmaxf(0.f, 1.f - fabsf(yabs - yv));
           const float we = whc * fmaxf(0.f, 1.f - fabsf(fminf(fmaxf(yabs, -cy), cy) - yv));
-          if (wm == 0.f && we == 0.f)
+          const float val = __uint_as_float(ve[r].x);
+          const int   ns = int(ve[r].y & 0xffffu), fs = int(ve[r].y >> 16);
+          if (jv < 0 || jv >= nv || kidx < min(ns, fs) || kidx > max(ns, fs))
             continue;
-          const size_t pix = size_t(p) * viewRays + size_t(jv) * nu + iu;
-          const uint2  ve = gend[pix];
-          const float  val = __uint_as_float(ve.x);
-          if (val == 0.f)
-            continue;
-          const unsigned en = ve.y;
-          const int      ns = int(en & 0xffffu), fs = int(en >> 16);
-          if (kidx < min(ns, fs) || kidx > max(ns, fs))
-            continue;
-          float sk = 1.f, w = wm;
+          float w = wm;
           if (kidx == ns || kidx == fs)
           {
-            const float2 w2 = gw[pix];
-            sk = kidx == ns ? w2.x : w2.y;
-            w = we;
+            const float2 w2 = gw[colBase + size_t(jv) * nu];
+            w = (kidx == ns ? w2.x : w2.y) * we;
           }
-          acc = fmaf(sk * w, val, acc);
+          acc = fmaf(w, val, acc);
         }
       }
     }
