@@ -45,7 +45,10 @@ class ThreeDCircularProjectionGeometry:
     """rtk::ThreeDCircularProjectionGeometry (src/rtkThreeDCircularProjectionGeometry.cxx:55-153)."""
 
     def __init__(self):
-        self._h = lib().rtkhip_geometry_create()
+        # the library is held by the instance: at interpreter exit the module's own
+        # globals may already be cleared when the destructor runs
+        self._dll = lib()
+        self._h = self._dll.rtkhip_geometry_create()
         if not self._h:
             raise _lib.RtkHipError(_lib.RTKHIP_ERR_OUT_OF_MEMORY, "geometry allocation failed")
         self._rows = []
@@ -54,8 +57,9 @@ class ThreeDCircularProjectionGeometry:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h and _lib._lib is not None:
-            _lib._lib.rtkhip_geometry_destroy(h)
+        dll = getattr(self, "_dll", None)
+        if h and dll is not None:
+            dll.rtkhip_geometry_destroy(h)
             self._h = None
 
     def AddProjection(self, sid, sdd, gantryAngle, projOffsetX=0., projOffsetY=0., outOfPlaneAngle=0.,

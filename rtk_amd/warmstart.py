@@ -58,7 +58,10 @@ class WarmStartCNN:
     """Inference and training of the warm-start network (device-resident)."""
 
     def __init__(self, params=None, scale=50.0, seed=0):
-        self._h = lib().rtkhip_cnn_create()
+        # the library is held by the instance: at interpreter exit the module's own
+        # globals may already be cleared when the destructor runs
+        self._dll = lib()
+        self._h = self._dll.rtkhip_cnn_create()
         if not self._h:
             raise _lib.RtkHipError(_lib.RTKHIP_ERR_NO_DEVICE, lib().rtkhip_last_error().decode())
         self.scale = float(scale)
@@ -67,8 +70,9 @@ class WarmStartCNN:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h and _lib._lib is not None:
-            _lib._lib.rtkhip_cnn_destroy(h)
+        dll = getattr(self, "_dll", None)
+        if h and dll is not None:
+            dll.rtkhip_cnn_destroy(h)
             self._h = None
 
     def set_params(self, p):
