@@ -53,6 +53,13 @@ struct alignas(16) ViewCoef
 constexpr int kTileU = 64, kTileV = 48, kTileStride = 76; // staged detector footprint, pixels
 constexpr int kStagePasses = (kTileV + 15) / 16;           // 16 rows per staging pass
 constexpr int kPlaceViews = 64;                            // footprints placed per chunk, one per lane
+// A footprint whose every tap lies at least kEdge pixels inside the detector's
+// interpolation range (0 <= u, v and u < nu - 1, v < nv - 1) is flagged in the bit
+// kInside of its height: then every column of the block passes the border rule and
+// reads the staged rectangle, and the column loop runs without its per-voxel tests.
+// kEdge is far above the rounding of the per-voxel u and v (a few ulps of ~1e3).
+constexpr float kEdge = 1.f / 16.f;
+constexpr int   kInside = 1 << 16;
 // 48 rows hold a 16-voxel column's taps up to 2.6 detector rows per voxel (the
 // 512^3 <- 1024^2 FDK reaches 2.34); each extra staging pass costs ~1 VALU per update
 
@@ -93,11 +100,13 @@ FootprintLane(const ViewCoef & c, float xlo, float xhi, float ylo, float yhi, fl
   const float fvLo = floorf(fmaxf(vmin, -2.f)), fvHi = floorf(fminf(vmax, float(nv)));
   if (fuHi < 0.f || fuLo > float(nu - 2) || fvHi < 0.f || fvLo > float(nv - 2))
     return make_int4(0, 0, 0, 0);
-  const int u0 = max(int(fuLo) - 1, 0) & ~3;
-  const int v0 = max(int(fvLo) - 1, 0);
-  const int u1 = min(int(fuHi) + 2, nu - 1), v1 = min(int(fvHi) + 2, nv - 1);
-  const int W = u1 - u0 + 1, H = v1 - v0 + 1;
-  return make_int4(u0, v0, (W > kTileU || H > kTileV) ? -1 : W, H);
+  const int  u0 = max(int(fuLo) - 1, 0) & ~3;
+  const int  v0 = max(int(fvLo) - 1, 0);
+  const int  u1 = min(int(fuHi) + 2, nu - 1), v1 = min(int(fvHi) + 2, nv - 1);
+  const int  W = u1 - u0 + 1, H = v1 - v0 + 1;
+  const bool fits = W <= kTileU && H <= kTileV;
+  const bool inside = umin >= kEdge && umax <= float(nu - 1) - kEdge && vmin >= kEdge && vmax <= float(nv - 1) - kEdge;
+  return make_int4(u0, v0, fits ? W : -1, H | (fits && inside ? kInside : 0));
 }
 
 // Each thread accumulates a column of NJ voxels along y; a workgroup is 16 (x) x 16
@@ -170,6 +179,7 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
 
   float sreg[4 * kStagePasses];
   int   cu0, cv0, cW, cH; // footprint of the view being backprojected
+  bool  cIn;              // ... and whether it lies inside the detector (kInside)
   // VEC (nu a multiple of 4, 16-byte aligned stack): every lane loads its four 4-pixel
   // chunks, 16 bytes (8 for 16-bit stacks) each, with no branch, so the loaded
   // registers are never copied before the arithmetic that hides their latency.  With
@@ -230,17 +240,19 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
     if (q < nviews)
       rects[chunk & 1][t & 63] = FootprintLane(views[q], xlo, xhi, ylo, yhi, zlo, zhi, nu, nv);
   };
-  auto rectOf = [&](int q, int & u0, int & v0, int & W, int & H) {
+  auto rectOf = [&](int q, int & u0, int & v0, int & W, int & H, bool & in) {
     const int4 r = rects[(q / kPlaceViews) & 1][q % kPlaceViews];
     u0 = __builtin_amdgcn_readfirstlane(r.x);
     v0 = __builtin_amdgcn_readfirstlane(r.y);
     W = __builtin_amdgcn_readfirstlane(r.z);
-    H = __builtin_amdgcn_readfirstlane(r.w);
+    const int h = __builtin_amdgcn_readfirstlane(r.w);
+    H = h & (kInside - 1);
+    in = (h & kInside) != 0;
   };
   if (wave == 0)
     place(0);
   __syncthreads();
-  rectOf(0, cu0, cv0, cW, cH);
+  rectOf(0, cu0, cv0, cW, cH, cIn);
   load(0, cu0, cv0, cW, cH);
   store(0, cW, cH);
   __syncthreads();
@@ -248,9 +260,10 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
   {
     const int b = p & 1;
     int       nu0 = 0, nv0 = 0, nW = 0, nH = 0;
+    bool      nIn = false;
     if (p + 1 < nviews)
     {
-      rectOf(p + 1, nu0, nv0, nW, nH);
+      rectOf(p + 1, nu0, nv0, nW, nH, nIn);
       load(p + 1, nu0, nv0, nW, nH);
     }
     if (wave == 0 && p % kPlaceViews == 0 && p + kPlaceViews < nviews)
@@ -264,45 +277,62 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
       const float      w = c.c[8] + c.c[9] * x + c.c[11] * z;
       const float      rw = __builtin_amdgcn_rcpf(w); // 1 ulp
       const float      u = (c.c[0] + c.c[1] * x + c.c[3] * z) * rw;
-      const float      uf = floorf(u);
-      if (uf >= 0.f && uf < numax)
+      const float      vb = (c.c[4] + c.c[5] * x + c.c[7] * z + c.c[6] * y0) * rw;
+      const float      dv = c.c[6] * rw;
+      const float      wgt = FDK ? rw * rw : 1.f;
+      // The column's detector column pair and whether it runs from the staged
+      // rectangle with no per-voxel test.  When the block's footprint lies inside
+      // the detector (wave-uniform) every column does, with no test at all: u >= kEdge,
+      // so truncation is the floor and fract(u) is the exact u - floor(u).
+      int   lc = 0;
+      float fu = 0.f;
+      bool  fast = true, inU = true;
+      if (!(viewFast && cIn))
       {
-        const int    ui = int(uf);
-        const float  fu = u - uf;
-        const float  vb = (c.c[4] + c.c[5] * x + c.c[7] * z + c.c[6] * y0) * rw;
-        const float  dv = c.c[6] * rw;
-        const float  wgt = FDK ? rw * rw : 1.f;
-        const int    lc = ui - cu0;
-        const float * S = stage[b];
-        const TP *   pp = proj + long(p) * projStride + ui;
+        const float uf = floorf(u);
+        inU = uf >= 0.f && uf < numax;
+        lc = inU ? int(uf) - cu0 : 0;
+        fu = u - uf;
         // v is monotonic along the column (one FMA per voxel), so its two ends bound
         // every tap: when both pass the border rule and lie in the staged rectangle,
         // the whole column runs without per-voxel tests
-        const float  vlast = vb + float(NJ - 1) * dv;
-        const float  vmn = vb < vlast ? vb : vlast, vmx = vb < vlast ? vlast : vb;
+        const float vlast = vb + float(NJ - 1) * dv;
+        const float vmn = vb < vlast ? vb : vlast, vmx = vb < vlast ? vlast : vb;
         // one combined test (no short-circuit branches): the column lies inside the
         // staged rows [cv0, cv0 + cH - 1) and inside [0, nv - 1), and its detector
         // column pair inside the staged columns; cv0 >= 0 by construction
-        const bool   fast = viewFast & (unsigned(lc) < unsigned(cW - 1)) & (floorf(vmn) >= fvLo) &
-                          (floorf(vmx) < fvHi);
-        if (fast)
-        {
-          const int    off = lc - cv0 * kTileStride;
-          const float  w0 = wgt * (1.f - fu), w1 = wgt * fu;
+        fast = inU & viewFast & (unsigned(lc) < unsigned(cW - 1)) & (floorf(vmn) >= fvLo) & (floorf(vmx) < fvHi);
+      }
+      else
+      {
+        lc = int(u) - cu0;
+        fu = __builtin_amdgcn_fractf(u);
+      }
+      if (fast)
+      {
+        int off = lc - cv0 * kTileStride;
+        // opaque to the optimiser: otherwise it folds cv0 into every voxel's row index
+        // and replaces the one 24-bit multiply-add per voxel by four instructions
+        __asm__("" : "+v"(off));
+        const float * S = stage[b];
+        const float   w0 = wgt * (1.f - fu), w1 = wgt * fu;
 #pragma unroll
-          for (int jj = 0; jj < NJ; jj++)
-          {
-            const float   vv = vb + float(jj) * dv;
-            // vv >= 0 on this path: truncation is the floor, and fract(vv) = vv - floor(vv)
-            const float   fv = __builtin_amdgcn_fractf(vv);
-            // 24-bit multiply: the LDS offset is small and the 32 x 32 form is slower
-            const float * s0 = S + (int(__umul24(unsigned(int(vv)), unsigned(kTileStride))) + off);
-            const float   t0 = s0[0] + fv * (s0[kTileStride] - s0[0]);
-            const float   t1 = s0[1] + fv * (s0[kTileStride + 1] - s0[1]);
-            acc[jj] = fmaf(w1, t1, fmaf(w0, t0, acc[jj]));
-          }
+        for (int jj = 0; jj < NJ; jj++)
+        {
+          const float   vv = vb + float(jj) * dv;
+          // vv >= 0 on this path: truncation is the floor, and fract(vv) = vv - floor(vv)
+          const float   fv = __builtin_amdgcn_fractf(vv);
+          // 24-bit multiply: the LDS offset is small and the 32 x 32 form is slower
+          const float * s0 = S + (int(__umul24(unsigned(int(vv)), unsigned(kTileStride))) + off);
+          const float   t0 = s0[0] + fv * (s0[kTileStride] - s0[0]);
+          const float   t1 = s0[1] + fv * (s0[kTileStride + 1] - s0[1]);
+          acc[jj] = fmaf(w1, t1, fmaf(w0, t0, acc[jj]));
         }
-        else
+      }
+      else if (inU)
+      {
+        const float * S = stage[b];
+        const TP *    pp = proj + long(p) * projStride + (lc + cu0);
 #pragma unroll
         for (int jj = 0; jj < NJ; jj++)
         {
@@ -344,6 +374,7 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(const ViewCoef * __restrict
     cv0 = nv0;
     cW = nW;
     cH = nH;
+    cIn = nIn;
   }
   if (live)
   {
