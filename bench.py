@@ -46,8 +46,8 @@ SEED = 20260001
 # kernel changes): HBM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors
 # from a 1 GiB device copy) and VALU lane instructions per voxel update
 # (SQ_INSTS_VALU x 64 / updates).
-BP_TRAFFIC_BYTES = 1.557e11
-BP_VALU_PER_UPDATE = 13.24
+BP_TRAFFIC_BYTES = 1.554e11
+BP_VALU_PER_UPDATE = 12.58
 # MI355X VALU issue peak: 256 CUs x 4 SIMDs, one 64-lane instruction per SIMD every
 # 4 cycles at 2.4 GHz = 3.93e13 lane instructions per second
 VALU_PEAK_LANE_INSTR = 1024 * 2.4e9 / 4 * 64
