@@ -1119,13 +1119,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void j
             // the staged rows of the tile (a block's candidate rows lie in it: the tile
             // holds the widened block's footprint with kGRows + 1 rows of margin)
             const float * tcol = &tile[b][rowBase * kGStride + lc];
-            auto middle = [&](auto rowsTag) {
-              constexpr int R = decltype(rowsTag)::value;
+            // EDGE: the block holds voxels of the outer layers along y, which take their
+            // rays below; elsewhere the loop has no branch, so the LDS reads of several
+            // voxels are issued before their first use
+            auto middle = [&](auto rowsTag, auto edgeTag) {
+              constexpr int  R = decltype(rowsTag)::value;
+              constexpr bool EDGE = decltype(edgeTag)::value;
 #pragma unroll
               for (int jj = 0; jj < kGNJ; jj++)
               {
-                // voxels of the outer layers along y: below (uniform)
-                if (j0 + jj < 2 || j0 + jj > ny - 3)
+                if (EDGE && (j0 + jj < 2 || j0 + jj > ny - 3))
                   continue;
                 const float d0f = ceilf(fmaf(float(jj), rb, q0));
                 const int   d0 = int(d0f);
@@ -1155,10 +1158,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void j
                 acc[jj] = fmaf(wh, s, acc[jj]);
               }
             };
-            if (__all(fabsf(bf) >= 0.5f))
-              middle(std::integral_constant<int, 4>{});
+            const bool four = __all(fabsf(bf) >= 0.5f);
+            if (STAGED && !yshell)
+            {
+              if (four)
+                middle(std::integral_constant<int, 4>{}, std::false_type{});
+              else
+                middle(std::integral_constant<int, kGRows>{}, std::false_type{});
+            }
+            else if (four)
+              middle(std::integral_constant<int, 4>{}, std::true_type{});
             else
-              middle(std::integral_constant<int, kGRows>{});
+              middle(std::integral_constant<int, kGRows>{}, std::true_type{});
             if (yshell)
             {
               // rows 0, 1, ny - 2, ny - 1 (those in this block, each once): the step weight
