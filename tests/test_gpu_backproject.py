@@ -20,20 +20,24 @@ GEOMS = {
 }
 
 
-def _case(oracle, gname, seed=0, vsize=(37, 33, 29), index=(0, 0, 0)):
+def _case(oracle, gname, seed=0, vsize=(37, 33, 29), index=(0, 0, 0), nu=61):
     rows = GEOMS[gname]
     vsp, vorig = (3., 2.5, 3.5), (-53., -40., -49.)
-    nu, nv, n = 61, 47, len(rows)
+    nv, n = 47, len(rows)
     ssp, sorig = (3.2, 3.2), (-96., -73.)
     rng = np.random.default_rng(seed)
     proj = rng.normal(size=(n, nv, nu)).astype(np.float32)
     return rows, vsize, vsp, vorig, (nu, nv, n), ssp, sorig, proj
 
 
+# nu = 64: rows of whole 16-byte chunks, the staging path with vector loads.  The
+# volume's footprint crosses the detector's edges, so the column kernel's blocks are
+# both inside the detector (no per-voxel test) and across its border.
+@pytest.mark.parametrize("nu", [61, 64])
 @pytest.mark.parametrize("fdk", [False, True])
 @pytest.mark.parametrize("gname", list(GEOMS))
-def test_backproject_parity(oracle, hip, gname, fdk):
-    rows, vsize, vsp, vorig, ssize, ssp, sorig, proj = _case(oracle, gname)
+def test_backproject_parity(oracle, hip, gname, fdk, nu):
+    rows, vsize, vsp, vorig, ssize, ssp, sorig, proj = _case(oracle, gname, nu=nu)
     rng = np.random.default_rng(1)
     v0 = rng.normal(size=vsize[::-1]).astype(np.float32)
     ov, os_ = oracle.image(vsize, vorig, vsp), oracle.image(ssize, sorig + (0.,), ssp + (1.,))
