@@ -16,7 +16,7 @@ evaluated on held-out seeds as CG time to a relative residual ||b - A x|| / ||b|
 of 1e-3 from three starts: zero, FDK, CNN(FDK); the start's own cost (FDK, CNN) is
 included in its time.
 
-Usage: python -m rtk_amd.warmstart_train --train 320 --steps 8000 --eval 16 --out w.npz
+Usage: python -m rtk_amd.warmstart_train --train 500 --steps 8000 --eval 16 --out w.npz
        python -m rtk_amd.warmstart_train --scale-eval 1   (512^3 / 180 views, tiled CNN)
        python -m rtk_amd.warmstart_train --scale-train 32 --steps 4000 --scale-eval 1 --out w5.npz
 """
@@ -299,7 +299,7 @@ def evaluate_at_scale(net, seed, tol=1e-3, max_iter=300, tile=128, halo=8):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--train", type=int, default=320)
+    ap.add_argument("--train", type=int, default=500)
     ap.add_argument("--eval", type=int, default=16)
     ap.add_argument("--steps", type=int, default=8000)
     ap.add_argument("--crop", type=int, default=64)

@@ -4,7 +4,7 @@ Sparse-view workload (256^3 @1 mm from 90 noisy views of 512^2), held-out phanto
 seed, CG with the matched Joseph pair to a relative residual of 1e-3 from zero,
 from FDK and from CNN(FDK) with the committed weights.  The learned start must
 reach the tolerance in at most 0.65x the iterations of plain CG from zero (measured
-0.57x on this seed) and in less wall time (its own FDK, CNN and A X0 cost included),
+0.49x on this seed) and in less wall time (its own FDK, CNN and A X0 cost included),
 in fewer iterations than the FDK start, from a smaller residual than FDK's, and end
 at a volume at least as close to the zero-start end point, and closer to the phantom,
 than the FDK start's."""
