@@ -301,6 +301,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=500)
     ap.add_argument("--eval", type=int, default=16)
+    ap.add_argument("--eval-seed0", type=int, default=EVAL_SEED0,
+                    help="first evaluation seed (held-out test seeds from 5000; other ranges for model selection)")
+    ap.add_argument("--seed", type=int, default=0, help="network initialisation and crop order of --train")
     ap.add_argument("--steps", type=int, default=8000)
     ap.add_argument("--crop", type=int, default=64)
     ap.add_argument("--batch", type=int, default=4)
@@ -363,7 +366,7 @@ def main():
         t2 = time.perf_counter()
         del samples
         net.save(args.out)
-        rows = evaluate(net, g, [EVAL_SEED0 + i for i in range(args.eval)])
+        rows = evaluate(net, g, [args.eval_seed0 + i for i in range(args.eval)])
         rep = {"finetune_steps": args.finetune, "samples": args.ft_samples, "lam_img": args.lam_img,
                "lam_res": args.lam_res, "lr": args.lr, "cg_iter": args.cg_iter, "data_seconds": t1 - t0,
                "train_seconds": t2 - t1, "first": list(np.mean(stats[:20], axis=0)),
@@ -380,7 +383,7 @@ def main():
     if args.eval_only:
         from rtk_amd.warmstart import default_weights_path
         net = WarmStartCNN.load(args.weights or default_weights_path())
-        rows = evaluate(net, g, [EVAL_SEED0 + i for i in range(args.eval)])
+        rows = evaluate(net, g, [args.eval_seed0 + i for i in range(args.eval)])
         rep = {"workload": "CG 256^3 @1 mm <- 90 x 512^2 @0.8 mm, Poisson I0=1e4, held-out seeds", "eval": rows}
         for name in ("zero", "fdk", "cnn"):
             rep[f"median_{name}_seconds"] = float(np.median([r[name]["seconds"] for r in rows]))
@@ -399,13 +402,13 @@ def main():
             log(f"pairs {i + 1}/{args.train} ({time.perf_counter() - t0:.0f} s), last CG residual {r:.3g}")
     scale = float(1.0 / torch.stack([p[1].abs().mean() for p in pairs]).mean().item())
     log(f"input scale {scale:.4g}")
-    net = WarmStartCNN(scale=scale, seed=0)
+    net = WarmStartCNN(scale=scale, seed=args.seed)
     t1 = time.perf_counter()
-    losses = train(net, pairs, args.steps, args.crop, args.batch, args.lr)
+    losses = train(net, pairs, args.steps, args.crop, args.batch, args.lr, seed=args.seed)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     net.save(args.out)
-    rows = evaluate(net, g, [EVAL_SEED0 + i for i in range(args.eval)])
+    rows = evaluate(net, g, [args.eval_seed0 + i for i in range(args.eval)])
     rep = {"train_pairs": args.train, "steps": args.steps, "crop": args.crop, "batch": args.batch,
            "data_seconds": t1 - t0, "train_seconds": t2 - t1, "first_loss": float(np.mean(losses[:20])),
            "last_loss": float(np.mean(losses[-20:])), "scale": scale, "eval": rows}
