@@ -475,7 +475,7 @@ def main():
     ap.add_argument("--no-scale", action="store_true", help="skip the warm start at 512^3 / 180 views")
     ap.add_argument("--cg-tol", type=float, default=1e-3)
     ap.add_argument("--cg-iter", type=int, default=100)
-    ap.add_argument("--warm-seeds", type=int, default=4, help="held-out seeds for the warm-start measurement")
+    ap.add_argument("--warm-seeds", type=int, default=16, help="held-out seeds for the warm-start measurement")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--cg-only", action="store_true", help="skip the FDK measurement")
