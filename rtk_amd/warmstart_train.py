@@ -34,6 +34,7 @@ from rtk_amd.warmstart import WarmStartCNN
 
 NX, NU, NVIEWS, I0 = 256, 512, 90, 1e4
 TRAIN_SEED0, EVAL_SEED0 = 1000, 5000
+SELECT_SEED0 = 7000  # checkpoint selection (--select), disjoint from training and evaluation seeds
 
 
 def log(msg):
@@ -100,16 +101,52 @@ def random_crops(pairs, crop, batch, rng):
     return torch.stack(xs).contiguous(), torch.stack(ts).contiguous()
 
 
-def train(net, pairs, steps, crop=64, batch=2, lr=1e-3, seed=0):
+def train(net, pairs, steps, crop=64, batch=2, lr=1e-3, seed=0, val=None, val_every=500):
+    """Adam on random crops with a cosine learning rate.  With `val` (a function of the
+    network, lower is better) the parameters are scored every `val_every` steps and at
+    the end, and the best-scoring ones are kept."""
     rng = np.random.default_rng(seed)
     losses = []
+    best, best_params = float("inf"), None
     for s in range(steps):
         x, t = random_crops(pairs, crop, batch, rng)
         step_lr = lr * 0.5 * (1 + np.cos(np.pi * s / steps))  # cosine decay
         losses.append(net.train_step(x, t, lr=step_lr))
         if (s + 1) % 100 == 0:
             log(f"train step {s + 1}/{steps}: loss {np.mean(losses[-100:]):.4g}")
+        if val is not None and ((s + 1) % val_every == 0 or s + 1 == steps):
+            score = val(net)
+            log(f"step {s + 1}: validation score {score:.4g}")
+            if score < best:
+                best, best_params = score, net.params()
+    if best_params is not None:
+        net.set_params(best_params)
+        log(f"kept the parameters of validation score {best:.4g}")
     return losses
+
+
+def validation_sample(g, seed):
+    """(FDK volume, B = BP_Joseph(p), the stack image) of one seed."""
+    vol = volume_source()
+    proj = noisy_projections(g, seed)
+    x_fdk = fdk(g, proj, vol)
+    bp = rtk.HipJosephBackProjectionImageFilter()
+    bp.SetInput(0, vol)
+    bp.SetInput(1, proj)
+    bp.SetGeometry(g)
+    bp.Update()
+    return x_fdk.tensor, bp.GetOutput().tensor.clone(), proj
+
+
+def start_residual(net, g, samples):
+    """Median over validation samples (FDK volume, B = BP(p), stack) of the relative
+    residual ||B - A CNN(FDK)|| / ||B|| the network's start gives CG."""
+    r = []
+    for x_fdk, b, proj in samples:
+        x0 = volume_source().like(net(x_fdk).contiguous())
+        ax = apply_normal_operator(g, x0, proj)
+        r.append(float(torch.linalg.norm((b - ax).double()) / torch.linalg.norm(b.double())))
+    return float(np.median(r))
 
 
 def apply_normal_operator(g, x, stack_like):
@@ -304,6 +341,8 @@ def main():
     ap.add_argument("--eval-seed0", type=int, default=EVAL_SEED0,
                     help="first evaluation seed (held-out test seeds from 5000; other ranges for model selection)")
     ap.add_argument("--seed", type=int, default=0, help="network initialisation and crop order of --train")
+    ap.add_argument("--select", type=int, default=0,
+                    help="keep the training checkpoint whose start residual is lowest on this many seeds from 7000")
     ap.add_argument("--steps", type=int, default=8000)
     ap.add_argument("--crop", type=int, default=64)
     ap.add_argument("--batch", type=int, default=4)
@@ -403,8 +442,12 @@ def main():
     scale = float(1.0 / torch.stack([p[1].abs().mean() for p in pairs]).mean().item())
     log(f"input scale {scale:.4g}")
     net = WarmStartCNN(scale=scale, seed=args.seed)
+    val = None
+    if args.select:
+        samples = [validation_sample(g, SELECT_SEED0 + i) for i in range(args.select)]
+        val = lambda n: start_residual(n, g, samples)  # noqa: E731
     t1 = time.perf_counter()
-    losses = train(net, pairs, args.steps, args.crop, args.batch, args.lr, seed=args.seed)
+    losses = train(net, pairs, args.steps, args.crop, args.batch, args.lr, seed=args.seed, val=val)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     net.save(args.out)
