@@ -1,6 +1,6 @@
 """Host logic of the warm-start training driver (no GPU): random crops are drawn over
 the whole of each volume, whatever its shape, and pair input with target voxel for
-voxel."""
+voxel; with a validation score the training keeps its best-scoring parameters."""
 import numpy as np
 import torch
 
@@ -23,3 +23,34 @@ def test_random_crops_cover_each_volume_and_stay_paired():
             lo, hi = np.minimum(lo, (z, y, w)), np.maximum(hi, (z, y, w))
     # origins reach both ends of every axis
     assert np.all(lo == 0) and np.all(hi == np.array(shape) - 32)
+
+
+class _FakeNet:
+    """Stands in for the network: a parameter vector that every step moves by one."""
+
+    def __init__(self):
+        self.p = np.zeros(3, np.float32)
+
+    def train_step(self, x, t, lr):
+        self.p += 1.
+        return float(lr)
+
+    def params(self):
+        return self.p.copy()
+
+    def set_params(self, p):
+        self.p = np.asarray(p, np.float32).copy()
+
+
+def test_train_keeps_the_best_scoring_checkpoint():
+    vol = torch.zeros((40, 40, 40))
+    net = _FakeNet()
+    # scores at steps 3, 6, 9, 10 (the last step is always scored): lowest at step 6
+    scores = {3: 0.5, 6: 0.1, 9: 0.3, 10: 0.2}
+    losses = W.train(net, [(vol, vol)], 10, crop=8, batch=1, val=lambda n: scores[int(n.p[0])], val_every=3)
+    assert len(losses) == 10
+    assert np.all(net.p == 6.)
+    # without a score function the last parameters stay
+    net = _FakeNet()
+    W.train(net, [(vol, vol)], 10, crop=8, batch=1)
+    assert np.all(net.p == 10.)
