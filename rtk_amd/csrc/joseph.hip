@@ -1254,7 +1254,12 @@ __global__ __launch_bounds__(256) void joseph_gather_shell_kernel(
   const uint2 * __restrict__ gend, const float2 * __restrict__ gw, const int * __restrict__ shell, int nshell,
   float * vol, int nx, int ny, int nz)
 {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  // Workgroups are dealt to the 8 XCDs in turn; XCD x takes a contiguous run of the
+  // shell list instead of every 8th workgroup, so the voxels one XCD's L2 serves lie
+  // together and read overlapping rays
+  const unsigned nb = gridDim.x, full = nb / 8, rem = nb % 8, x = blockIdx.x % 8, q = blockIdx.x / 8;
+  const unsigned lb = x * full + min(x, rem) + q;
+  const int      n = int(lb * blockDim.x + threadIdx.x);
   if (n >= nshell)
     return;
   const int    idx = shell[n];
