@@ -1500,7 +1500,13 @@ __global__ __launch_bounds__(256) void joseph_fp_gather_kernel(
   // grid (column blocks, views, row bands), as the per-ray projector: workgroups in
   // flight share a band of detector rows, whose rays stay in one slab of the volume,
   // so the slab stays in L2 while every view reads it
-  const int        p = blockIdx.y, iu = blockIdx.x * 16 + c, rb = blockIdx.z * kFRows;
+  // workgroups are dealt to the 8 XCDs in turn in linear order; renumbered so that XCD
+  // x takes a contiguous run of (column block, view, row band) triples
+  const unsigned   nblk = gridDim.x * gridDim.y * gridDim.z, full = nblk / 8, rem = nblk % 8;
+  const unsigned   lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const unsigned   xcd = lin % 8, blk = xcd * full + min(xcd, rem) + lin / 8;
+  const int        p = int((blk / gridDim.x) % gridDim.y), iu = int(blk % gridDim.x) * 16 + c,
+                   rb = int(blk / (gridDim.x * gridDim.y)) * kFRows;
   const int        rlast = min(rb + kFRows, nv) - 1;
   const int        jv0 = rb + rq; // this thread's rays: rows jv0 + 16 q
   if (t < 2)
