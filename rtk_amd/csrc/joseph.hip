@@ -1500,14 +1500,24 @@ __global__ __launch_bounds__(256) void joseph_fp_gather_kernel(
   // grid (column blocks, views, row bands), as the per-ray projector: workgroups in
   // flight share a band of detector rows, whose rays stay in one slab of the volume,
   // so the slab stays in L2 while every view reads it
-  // workgroups are dealt to the 8 XCDs in turn in linear order.  With xcdOrder they are
-  // renumbered so that XCD x takes a contiguous run of (column block, view, row band)
-  // triples: its L2 then serves one band's rays, and the 8 XCDs read 8 bands' slabs
-  // of the volume at once (the host asks for it when both volume copies fit the
-  // last-level cache)
+  // Workgroups are dealt to the 8 XCDs in turn in linear order.  xcdOrder 1 renumbers
+  // them so that XCD x takes a contiguous run of (column block, view, row band)
+  // triples: its L2 then serves one band's rays, and the 8 XCDs read 8 bands' slabs of
+  // the volume at once (for volumes whose two copies fit the last-level cache).
+  // xcdOrder 2 keeps the bands in order and, within a band, gives XCD x a contiguous
+  // run of its (column block, view) pairs: all XCDs read one slab, each for a range of
+  // neighbouring views.
   const unsigned   nblk = gridDim.x * gridDim.y * gridDim.z, full = nblk / 8, rem = nblk % 8;
   const unsigned   lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-  const unsigned   xcd = lin % 8, blk = xcdOrder ? xcd * full + min(xcd, rem) + lin / 8 : lin;
+  unsigned         blk = lin;
+  if (xcdOrder == 1)
+    blk = (lin % 8) * full + min(lin % 8, rem) + lin / 8;
+  else if (xcdOrder == 2)
+  {
+    // within each row band: XCD x takes a contiguous run of its (column block, view) pairs
+    const unsigned P = gridDim.x * gridDim.y, w = lin % P, f = P / 8, r = P % 8, x = w % 8;
+    blk = (lin / P) * P + x * f + min(x, r) + w / 8;
+  }
   const int        p = int((blk / gridDim.x) % gridDim.y), iu = int(blk % gridDim.x) * 16 + c,
                    rb = int(blk / (gridDim.x * gridDim.y)) * kFRows;
   const int        rlast = min(rb + kFRows, nv) - 1;
@@ -1972,10 +1982,11 @@ rtkhip_joseph_forward_project(const int proj_dim[3], const int vol_dim[3], const
       if (nblocks >= 0 && forward)
       {
         dim3 fg((nu + 15) / 16, np, (nv + kFRows - 1) / kFRows);
-        // XCD-contiguous workgroups when the volume and its transposed copy fit well in
-        // the 256 MB last-level cache: 256^3 (2 x 67 MB) 5.50 -> 4.71 ms, while 512^3
-        // (2 x 537 MB) took 318 -> 340 ms that way
-        const int xcdOrder = 2 * size_t(vol_dim[0]) * vol_dim[1] * vol_dim[2] * sizeof(float) <= (size_t(192) << 20);
+        // XCD-contiguous workgroups over the whole grid when the volume and its
+        // transposed copy fit well in the 256 MB last-level cache (256^3 <- 90 x 512^2:
+        // 5.50 -> 4.73 ms; 4.82 ms within bands), within each row band otherwise
+        // (512^3 <- 720 x 1024^2: 318 -> 283 ms; 340 ms over the whole grid)
+        const int xcdOrder = 2 * size_t(vol_dim[0]) * vol_dim[1] * vol_dim[2] * sizeof(float) <= (size_t(192) << 20) ? 1 : 2;
         hipLaunchKernelGGL(joseph_fp_gather_kernel, fg, dim3(256), 0, s, dv, g_joseph.gviews[s].get(),
                            g_joseph.gcols[s].get(), g_joseph.gaxis[s].get(), nu, nv, dev_proj_in, dev_proj_out,
                            dev_vol, dT.get(), vol_dim[0], vol_dim[1], vol_dim[2], vol_spacing[0], vol_spacing[1],
